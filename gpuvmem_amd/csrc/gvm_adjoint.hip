@@ -1,0 +1,447 @@
+// DChi2 adjoint on gfx950 MFMA.
+//
+// Reference: DChi2 (src/functions.cu:3698-3791) evaluates, for every pixel
+// inside the mask,
+//     d(i,j) = -fg_scale PB(i,j) sum_v w_v (Re Vr_v cos phi + Im Vr_v sin phi),
+//     phi    = 2 pi (x_j u_v + y_i v_v + (z_ij - 1) w_v),
+// i.e. d = -fg PB Re sum_v c_v e^{i phi} with c_v = w_v conj(Vr_v): a direct
+// DFT over all visibilities for each pixel (M N Nvis sincos).
+//
+// Here the phase is split per axis, e^{i phi} = Ex_v(j) Ey_v(i) with
+//     Ex_v(j) = e^{2 pi i (u_v x_j + w_v f(x_j))},  f(t) = sqrt(1 - t^2) - 1,
+// (the neglected cross term of the w phase is about w x^2 y^2 / 4 turns), so
+//     G(i,j) = Re sum_v [c_v Ey_v(i)] Ex_v(j) = sum_k P(i,k) Q(k,j)
+// is a real GEMM over k = (v, re/im): P = [Re(c Ey), -Im(c Ey)],
+// Q = [Re Ex, Im Ex].  P and Q are generated per chunk of visibilities into
+// LDS (they are never materialised in HBM) and multiplied on the MFMA units:
+//   F32    v_mfma_f32_32x32x2_f32 on FP32 operands;
+//   F16X3  FP16 hi/lo split of both operands, P_hi Q_hi + P_hi Q_lo + P_lo Q_hi
+//          with v_mfma_f32_32x32x16_f16, FP32 accumulation;
+//   BF16X3 the same with BF16 (v_mfma_f32_32x32x16_bf16).
+// P carries c scaled by a power of two (from the forward pass' max |c|) so the
+// FP16 operands stay in range.  The visibilities are split over K into
+// `nsplit` ranges; each workgroup owns one 256x256 output tile and one range
+// and writes an FP32 partial tile; the epilogue sums the partials in FP64,
+// applies -fg PB (and 1/nvis) on the mask and the chain rule of
+// DChi2_total_I_nu_0 / DChi2_total_alpha (src/functions.cu:4000-4024,
+// 3968-3998).
+//
+// Phase arguments are kept exact in turns: u dx = a_hi + a_lo with a_hi on a
+// 2^-(23 - log2 N) grid, so a_hi * n is exact in FP32 and its fraction is
+// exact; the remainder a_lo n + w f(x) is small.  The FP64 split is done once
+// at set-up.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "gvm_internal.h"
+
+namespace gvm {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+
+enum { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2 };
+
+constexpr int kTile = 256;  // output tile edge (rows and columns)
+constexpr int kKV = 16;     // visibilities per chunk
+constexpr int kKP = 2 * kKV;
+
+// sin and cos of 2 pi t (t in turns).  v_sin_f32 / v_cos_f32 take their
+// argument in turns, valid on [-256, 256]; |t| < 2 here.
+__device__ __forceinline__ void sincos_turns(float t, float* s, float* c) {
+  *s = __builtin_amdgcn_sinf(t);
+  *c = __builtin_amdgcn_cosf(t);
+}
+
+__device__ __forceinline__ float fract(float x) { return __builtin_amdgcn_fractf(x); }
+
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) {
+  return __uint_as_float(((unsigned int)h) << 16);
+}
+
+template <int PREC>
+struct AdjLds;
+
+template <>
+struct AdjLds<PREC_F32> {
+  static constexpr int kStride = kKP + 4;  // floats per row (144 B)
+  float a[kTile * kStride];
+  float b[kTile * kStride];
+};
+
+template <int PREC>
+struct AdjLds {
+  static constexpr int kStride = kKP + 8;  // 16-bit elements per row (80 B)
+  unsigned short ah[kTile * kStride];
+  unsigned short al[kTile * kStride];
+  unsigned short bh[kTile * kStride];
+  unsigned short bl[kTile * kStride];
+};
+
+// Splits x into hi + lo in the 16-bit format of PREC.
+template <int PREC>
+__device__ __forceinline__ void split16(float x, unsigned short* hi, unsigned short* lo) {
+  if (PREC == PREC_F16) {
+    _Float16 h = (_Float16)x;
+    _Float16 l = (_Float16)(x - (float)h);
+    *hi = __builtin_bit_cast(unsigned short, h);
+    *lo = __builtin_bit_cast(unsigned short, l);
+  } else {
+    unsigned short h = bf16_rne(x);
+    *hi = h;
+    *lo = bf16_rne(x - bf16_to_f(h));
+  }
+}
+
+__device__ __forceinline__ f32x16_t mfma16(const u16x8_t& a, const u16x8_t& b, f32x16_t c,
+                                           int prec) {
+  if (prec == PREC_F16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8_t, a),
+                                                  __builtin_bit_cast(half8_t, b), c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                 __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+}
+
+// One workgroup: 256 threads (4 waves) own a 256x256 tile of G for one
+// K-range.  Wave w computes rows (w/2)*128.. and columns (w%2)*128.. as 4x4
+// blocks of 32x32 MFMA outputs (256 FP32 accumulators per lane).
+template <int PREC>
+__global__ __launch_bounds__(256, 1) void adjoint_gemm_kernel(
+    const float4* __restrict__ phs, const float* __restrict__ wcoord,
+    const float* __restrict__ weight, const float2* __restrict__ vr, long nvis,
+    const float2* __restrict__ xtab, const float2* __restrict__ ytab, int N, int tiles_x,
+    int ntiles, int nsplit, long kper, const unsigned int* __restrict__ cmax,
+    float* __restrict__ part) {
+  __shared__ AdjLds<PREC> sm;
+  constexpr int S = AdjLds<PREC>::kStride;
+
+  // XCD-aware order: the workgroups of one K-range (which read the same
+  // visibilities) are dispatched to the same XCD (blockIdx % 8) and share its L2.
+  const int nwg = ntiles * nsplit;
+  const int per_xcd = nwg / 8;
+  const int lin = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  const int split = lin / ntiles;
+  const int tile = lin % ntiles;
+  const int ti = tile / tiles_x, tj = tile % tiles_x;
+
+  const long kbeg = (long)split * kper;
+  const long kend = min(nvis, kbeg + kper);
+
+  const int t = threadIdx.x;
+  const int row = ti * kTile + t, col = tj * kTile + t;
+  const bool rvalid = row < N, cvalid = col < N;
+  const float2 yt = rvalid ? ytab[row] : make_float2(0.f, 0.f);
+  const float2 xt = cvalid ? xtab[col] : make_float2(0.f, 0.f);
+
+  const float cm = __uint_as_float(*cmax);
+  const float scale = cm > 0.0f ? exp2f(-ceilf(log2f(cm))) : 1.0f;
+
+  const int lane = t & 63, wave = t >> 6;
+  const int lr = lane & 31, lh = lane >> 5;
+  const int wr = (wave >> 1) * 128, wc = (wave & 1) * 128;
+
+  f32x16_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[a][b][e] = 0.0f;
+
+  for (long k0 = kbeg; k0 < kend; k0 += kKV) {
+    // ---- generate P (row t) and Q (column t) for kKV visibilities ----
+    float pv[kKP], qv[kKP];
+#pragma unroll
+    for (int kk = 0; kk < kKV; kk++) {
+      const long k = k0 + kk;
+      float4 ph = make_float4(0.f, 0.f, 0.f, 0.f);
+      float wcv = 0.f, cr = 0.f, ci = 0.f;
+      if (k < kend) {
+        ph = phs[k];
+        wcv = wcoord[k];
+        const float wt = weight[k] * scale;
+        const float2 r = vr[k];
+        cr = wt * r.x;
+        ci = -wt * r.y;
+      }
+      float sy, cy, sx, cx;
+      const float ty = fract(ph.z * yt.x) + fmaf(ph.w, yt.x, wcv * yt.y);
+      sincos_turns(ty, &sy, &cy);
+      const float tx = fract(ph.x * xt.x) + fmaf(ph.y, xt.x, wcv * xt.y);
+      sincos_turns(tx, &sx, &cx);
+      const float pr = cr * cy - ci * sy;
+      const float pim = cr * sy + ci * cy;
+      pv[2 * kk] = rvalid ? pr : 0.0f;
+      pv[2 * kk + 1] = rvalid ? -pim : 0.0f;
+      qv[2 * kk] = cx;
+      qv[2 * kk + 1] = sx;
+    }
+    if constexpr (PREC == PREC_F32) {
+#pragma unroll
+      for (int e = 0; e < kKP; e += 4) {
+        *(float4*)&sm.a[t * S + e] = make_float4(pv[e], pv[e + 1], pv[e + 2], pv[e + 3]);
+        *(float4*)&sm.b[t * S + e] = make_float4(qv[e], qv[e + 1], qv[e + 2], qv[e + 3]);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < kKP; e += 8) {
+        u16x8_t ph8, pl8, qh8, ql8;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          unsigned short h, l;
+          split16<PREC>(pv[e + q], &h, &l);
+          ph8[q] = h;
+          pl8[q] = l;
+          split16<PREC>(qv[e + q], &h, &l);
+          qh8[q] = h;
+          ql8[q] = l;
+        }
+        *(u16x8_t*)&sm.ah[t * S + e] = ph8;
+        *(u16x8_t*)&sm.al[t * S + e] = pl8;
+        *(u16x8_t*)&sm.bh[t * S + e] = qh8;
+        *(u16x8_t*)&sm.bl[t * S + e] = ql8;
+      }
+    }
+    __syncthreads();
+
+    // ---- MFMA over the chunk ----
+    if constexpr (PREC == PREC_F32) {
+      // lane (row lr, group lh) takes k = lh*16 + s at step s
+      float av[4][16], bv[4][16];
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) {
+        const float* src = &sm.a[(wr + rb * 32 + lr) * S + lh * 16];
+#pragma unroll
+        for (int e = 0; e < 16; e += 4) {
+          const float4 v = *(const float4*)&src[e];
+          av[rb][e] = v.x;
+          av[rb][e + 1] = v.y;
+          av[rb][e + 2] = v.z;
+          av[rb][e + 3] = v.w;
+        }
+      }
+#pragma unroll
+      for (int cb = 0; cb < 4; cb++) {
+        const float* src = &sm.b[(wc + cb * 32 + lr) * S + lh * 16];
+#pragma unroll
+        for (int e = 0; e < 16; e += 4) {
+          const float4 v = *(const float4*)&src[e];
+          bv[cb][e] = v.x;
+          bv[cb][e + 1] = v.y;
+          bv[cb][e + 2] = v.z;
+          bv[cb][e + 3] = v.w;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 16; s++)
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+          for (int cb = 0; cb < 4; cb++)
+            acc[rb][cb] =
+                __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb][s], bv[cb][s], acc[rb][cb], 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int s = 0; s < kKP / 16; s++) {
+        // lane (row lr, group lh) takes k = s*16 + lh*8 + e
+        u16x8_t ahi[4], alo[4], bhi[4], blo[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+          const int o = (wr + rb * 32 + lr) * S + s * 16 + lh * 8;
+          ahi[rb] = *(const u16x8_t*)&sm.ah[o];
+          alo[rb] = *(const u16x8_t*)&sm.al[o];
+        }
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+          const int o = (wc + cb * 32 + lr) * S + s * 16 + lh * 8;
+          bhi[cb] = *(const u16x8_t*)&sm.bh[o];
+          blo[cb] = *(const u16x8_t*)&sm.bl[o];
+        }
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+          for (int cb = 0; cb < 4; cb++) {
+            acc[rb][cb] = mfma16(ahi[rb], bhi[cb], acc[rb][cb], PREC);
+            acc[rb][cb] = mfma16(ahi[rb], blo[cb], acc[rb][cb], PREC);
+            acc[rb][cb] = mfma16(alo[rb], bhi[cb], acc[rb][cb], PREC);
+          }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- write the partial tile: 32x32 accumulator layout, lane l, reg e:
+  //      row 8*(e/4) + 4*(l/32) + e%4, column l%32 ----
+  float* out = part + (long)split * N * N;
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        const int r = ti * kTile + wr + rb * 32 + 8 * (e >> 2) + 4 * lh + (e & 3);
+        const int c = tj * kTile + wc + cb * 32 + lr;
+        if (r < N && c < N) out[(long)r * N + c] = acc[rb][cb][e];
+      }
+}
+
+// Direct sum (the reference's algorithm, exact w term), one thread per pixel,
+// visibilities staged through LDS.  FP32 terms, FP64 accumulation per 256
+// visibilities.  Writes G to part[0].
+__global__ __launch_bounds__(256) void adjoint_direct_kernel(
+    const float4* __restrict__ phs, const float* __restrict__ wcoord,
+    const float* __restrict__ weight, const float2* __restrict__ vr, long nvis,
+    const float2* __restrict__ xtab, const float2* __restrict__ ytab, int N, double dx_rad,
+    double dy_rad, float* __restrict__ part) {
+  __shared__ float4 sp[256];
+  __shared__ float4 sc[256];  // (w, cr, ci, 0)
+  const long pix = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool valid = pix < (long)N * N;
+  const int i = valid ? (int)(pix / N) : 0, j = valid ? (int)(pix % N) : 0;
+  const float2 xt = xtab[j], yt = ytab[i];
+  const double x = xt.x * dx_rad, y = yt.x * dy_rad;
+  const float zm1 = (float)(-(x * x + y * y) / (1.0 + sqrt(1.0 - x * x - y * y)));
+  double acc = 0.0;
+  for (long k0 = 0; k0 < nvis; k0 += 256) {
+    const long k = k0 + threadIdx.x;
+    __syncthreads();
+    if (k < nvis) {
+      sp[threadIdx.x] = phs[k];
+      const float2 r = vr[k];
+      const float w = weight[k];
+      sc[threadIdx.x] = make_float4(wcoord[k], w * r.x, -w * r.y, 0.f);
+    } else {
+      sp[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+      sc[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    float part_acc = 0.0f;
+    for (int q = 0; q < 256; q++) {
+      const float4 p = sp[q];
+      const float4 c = sc[q];
+      const float t = fract(p.x * xt.x) + fract(p.z * yt.x) +
+                      fmaf(p.y, xt.x, fmaf(p.w, yt.x, c.x * zm1));
+      float s, co;
+      sincos_turns(t, &s, &co);
+      part_acc += c.y * co - c.z * s;
+    }
+    acc += (double)part_acc;
+  }
+  if (valid) part[pix] = (float)acc;
+}
+
+// Sum of the split-K partials (FP64), -fg PB on the mask, 1/nvis if
+// normalize, then the chain rule into result.
+__global__ void adjoint_epilogue_kernel(const float* __restrict__ part, int nsplit, long n,
+                                        const unsigned int* __restrict__ cmax, int scaled,
+                                        const float* __restrict__ noise, float noise_cut,
+                                        const float* __restrict__ pb, float fg_scale,
+                                        double inv_nvis, float nudiv, float threshold,
+                                        int flag_opt, const float* __restrict__ I,
+                                        float* __restrict__ result) {
+  double inv_scale = 1.0;
+  if (scaled) {
+    const float cm = __uint_as_float(*cmax);
+    inv_scale = cm > 0.0f ? (double)exp2f(ceilf(log2f(cm))) : 1.0;
+  }
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x) {
+    if (!(noise[k] < noise_cut)) continue;
+    double g = 0.0;
+    for (int s = 0; s < nsplit; s++) g += (double)part[(long)s * n + k];
+    // DChi2: dchi2 = -(sum * fg_scale * atten) [/ nvis]
+    const float d = (float)(-(g * inv_scale) * (double)(fg_scale * pb[k]) * inv_nvis);
+    const float i0 = I[k], alpha = I[n + k];
+    const float dI = powf(nudiv, alpha);
+    if (flag_opt % 2 == 0) {
+      result[k] += d * dI;
+    } else if (i0 > threshold) {
+      result[n + k] += d * (i0 * dI * fg_scale * logf(nudiv));
+    }
+  }
+}
+
+namespace {
+
+int choose_nsplit(long nvis, int ntiles, int cu_count) {
+  long target_wg = 4L * std::max(cu_count, 64);
+  long ns = (target_wg + ntiles - 1) / ntiles;
+  ns = std::max(ns, (nvis + 32767) / 32768);  // bound the FP32 accumulation length
+  ns = std::min(ns, std::max(1L, nvis / 512));
+  ns = std::max(ns, 1L);
+  ns = ((ns + 7) / 8) * 8;  // the XCD remap needs multiples of 8 workgroups
+  return (int)std::min(ns, 1024L);
+}
+
+}  // namespace
+
+gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, int normalize) {
+  const gvm_geometry& g = ctx->g;
+  const int N = g.N;
+  const long n = (long)N * N;
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  const int tiles_x = (N + kTile - 1) / kTile;
+  const int ntiles = tiles_x * tiles_x;
+  for (size_t si = 0; si < ctx->slices.size(); si++) {
+    const Slice& s = ctx->slices[si];
+    const int mode = ctx->adjoint_mode;
+    const int nsplit = mode == GVM_ADJOINT_DIRECT ? 1 : choose_nsplit(s.nvis, ntiles, ctx->cu_count);
+    const size_t need = (size_t)nsplit * n;
+    if (need > ctx->adj_part_elems) {
+      GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+      if (ctx->adj_part) GVM_HIP_TRY(hipFree(ctx->adj_part));
+      ctx->adj_part = nullptr;
+      ctx->adj_part_elems = 0;
+      GVM_HIP_TRY(hipMalloc((void**)&ctx->adj_part, sizeof(float) * need));
+      ctx->adj_part_elems = need;
+    }
+    const long kper = (((s.nvis + nsplit - 1) / nsplit) + kKV - 1) / kKV * kKV;
+    GVM_HIP_TRY(hipEventRecord(ctx->ev_a, ctx->stream));
+    if (mode == GVM_ADJOINT_DIRECT) {
+      const double dxr = g.deltax * 3.14159265358979323846 / 180.0;
+      const double dyr = g.deltay * 3.14159265358979323846 / 180.0;
+      hipLaunchKernelGGL(adjoint_direct_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                         ctx->stream, s.phs, s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab,
+                         ctx->ytab, N, dxr, dyr, ctx->adj_part);
+    } else {
+      const dim3 grid(ntiles * nsplit);
+      if (mode == GVM_ADJOINT_F32)
+        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F32>, grid, dim3(256), 0, ctx->stream, s.phs,
+                           s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
+                           ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
+      else if (mode == GVM_ADJOINT_F16X3)
+        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F16>, grid, dim3(256), 0, ctx->stream, s.phs,
+                           s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
+                           ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
+      else
+        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_BF16>, grid, dim3(256), 0, ctx->stream, s.phs,
+                           s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
+                           ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
+    }
+    GVM_CHECK_LAUNCH();
+    GVM_HIP_TRY(hipEventRecord(ctx->ev_b, ctx->stream));
+    const double inv_nvis = normalize ? 1.0 / (double)s.nvis : 1.0;
+    hipLaunchKernelGGL(adjoint_epilogue_kernel, dim3(reduce_grid(n)), dim3(256), 0, ctx->stream,
+                       ctx->adj_part, nsplit, n, ctx->cmax + si,
+                       mode == GVM_ADJOINT_DIRECT ? 0 : 1, ctx->noise, g.noise_cut, s.pb,
+                       g.fg_scale, inv_nvis, s.freq / g.nu_0, g.threshold, flag_opt, I, result);
+    GVM_CHECK_LAUNCH();
+    GVM_HIP_TRY(hipEventSynchronize(ctx->ev_b));
+    float ms = 0.f;
+    GVM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
+    ctx->adj_ms_total += ms;
+    ctx->adj_count += 1;
+  }
+  return GVM_OK;
+}
+
+}  // namespace gvm

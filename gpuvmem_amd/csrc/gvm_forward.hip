@@ -1,0 +1,314 @@
+// Forward model and chi2 on gfx950.
+//
+// Per slice (channel x stokes) the reference runs calculateInu, apply_beam2I,
+// cuFFT C2C (inverse, unnormalised), phase_rotate, vis_mod, residual and
+// chi2Vector + a float tree reduction as eight separate launches with a device
+// synchronisation after each (src/functions.cu:4323-4454).  Here:
+//
+//  1. clip_kernel        clip2IWNoise in place (once per evaluation);
+//  2. fft_pass<rows>     I_nu = clip(I0 (nu/nu0)^alpha), times PB and fg_scale,
+//                        then the row transforms (Stockham radix-4/2 in LDS),
+//                        times the column-frequency phase-rotation factor;
+//  3. fft_pass<cols>     column transforms in LDS, times the row-frequency
+//                        phase-rotation factor, in place on the grid;
+//  4. degrid_chi2        bilinear 4-tap gather, V_r = V_o - V_m, w |V_r|^2
+//                        summed per block in FP64, and max |w V_r| for the
+//                        adjoint's FP16 scaling;
+//  5. finish_kernel      fixed-order FP64 sum of the block partials.
+//
+// The transform is e^{+2 pi i} (CUFFT_INVERSE) and unnormalised.  The phase
+// rotation e^{-2 pi i (u x0/M + v y0/N)} (src/functions.cu:2483-2518)
+// factorises per axis, so each pass applies its own factor to its output.
+#include <hip/hip_runtime.h>
+
+#include "gvm_internal.h"
+
+namespace gvm {
+
+// ---------------------------------------------------------------------------
+// clip2IWNoise: src/functions.cu:2694-2719
+// ---------------------------------------------------------------------------
+__global__ void clip_kernel(float* __restrict__ I, const float* __restrict__ noise, long n,
+                            float noise_cut, float floor_v, int eta_pos, float threshold,
+                            int flag_opt) {
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x) {
+    if (noise[k] > noise_cut) {
+      I[k] = eta_pos ? 0.0f : floor_v;
+      I[n + k] = 0.0f;
+    } else if (I[k] < threshold && flag_opt > 0) {
+      I[n + k] = 0.0f;
+    }
+  }
+}
+
+__global__ void zero_u32(unsigned int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Stockham FFT in LDS
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// One Stockham stage (radix R, current sub-transform length NS) over B
+// transforms of length N held in LDS.  COLS selects the LDS layout:
+//   rows: element (t, p) at t * N + p;  cols: element (t, p) at p * B + t.
+// All reads of the stage complete (barrier) before any write, so the stage
+// runs in place in one buffer.
+template <int N, int B, bool COLS, int R, int NS>
+__device__ __forceinline__ void stockham_stage(float2* lds, const float2* __restrict__ tw) {
+  constexpr int NB = B * N / R;          // butterflies in this stage
+  constexpr int BPT = NB / 256;          // per thread
+  static_assert(NB % 256 == 0, "butterflies must divide evenly over 256 threads");
+  constexpr int STRIDE = N / R;
+  float2 v[BPT][R];
+  int base_t[BPT], out_j[BPT];
+#pragma unroll
+  for (int b = 0; b < BPT; b++) {
+    const int q = threadIdx.x + b * 256;
+    const int t = COLS ? (q % B) : (q / STRIDE);
+    const int j = COLS ? (q / B) : (q % STRIDE);
+    const int k = j % NS;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int p = j + r * STRIDE;
+      v[b][r] = COLS ? lds[p * B + t] : lds[t * N + p];
+    }
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        const int e = r * k * (N / (NS * R));
+        v[b][r] = cmul(v[b][r], tw[e]);
+      }
+    }
+    if (R == 4) {
+      // b_m = sum_r a_r e^{+2 pi i r m / 4}
+      const float2 a0 = v[b][0], a1 = v[b][1], a2 = v[b][2], a3 = v[b][3];
+      const float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
+      const float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
+      const float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
+      const float2 d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
+      v[b][0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+      v[b][2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+      // i * d13 = (-d13.y, d13.x)
+      v[b][1] = make_float2(d02.x - d13.y, d02.y + d13.x);
+      v[b][3] = make_float2(d02.x + d13.y, d02.y - d13.x);
+    } else {
+      const float2 a0 = v[b][0], a1 = v[b][1];
+      v[b][0] = make_float2(a0.x + a1.x, a0.y + a1.y);
+      v[b][1] = make_float2(a0.x - a1.x, a0.y - a1.y);
+    }
+    base_t[b] = t;
+    out_j[b] = (j / NS) * NS * R + k;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; b++) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int p = out_j[b] + r * NS;
+      if (COLS)
+        lds[p * B + base_t[b]] = v[b][r];
+      else
+        lds[base_t[b] * N + p] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int B, bool COLS, int NS>
+__device__ __forceinline__ void radix4_stages(float2* lds, const float2* __restrict__ tw) {
+  if constexpr (NS < N) {
+    stockham_stage<N, B, COLS, 4, NS>(lds, tw);
+    radix4_stages<N, B, COLS, NS * 4>(lds, tw);
+  }
+}
+
+template <int N>
+constexpr bool log2_is_odd() {
+  int l = 0;
+  int n = N;
+  while (n > 1) {
+    n >>= 1;
+    l++;
+  }
+  return (l & 1) != 0;
+}
+
+template <int N, int B, bool COLS>
+__device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ tw) {
+  if constexpr (log2_is_odd<N>()) {
+    stockham_stage<N, B, COLS, 2, 1>(lds, tw);
+    radix4_stages<N, B, COLS, 2>(lds, tw);
+  } else {
+    radix4_stages<N, B, COLS, 1>(lds, tw);
+  }
+}
+
+// Row pass: build I_nu * PB * fg_scale for B rows (calculateInu +
+// apply_beam2I, src/functions.cu:3939-3966, 2424-2444), transform each row,
+// multiply by rot_x[column frequency], write the rows of the grid.
+template <int N, int B>
+__global__ __launch_bounds__(256) void fft_rows_kernel(
+    const float* __restrict__ I, const float* __restrict__ pb, float fg_scale, float nudiv,
+    float floor_v, const float2* __restrict__ tw, const float2* __restrict__ rot_x,
+    float2* __restrict__ grid) {
+  __shared__ float2 lds[B * N];
+  const long n = (long)N * N;
+  const int row0 = blockIdx.x * B;
+  for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+    const long k = (long)(row0 + idx / N) * N + (idx % N);
+    float inu = I[k] * powf(nudiv, I[n + k]);
+    if (inu < floor_v) inu = floor_v;
+    lds[idx] = make_float2(inu * pb[k] * fg_scale, 0.0f);
+  }
+  __syncthreads();
+  fft_lds<N, B, false>(lds, tw);
+  for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+    const int j = idx % N;
+    grid[(long)(row0 + idx / N) * N + j] = cmul(lds[idx], rot_x[j]);
+  }
+}
+
+// Column pass: B adjacent columns per block, in place on the grid.
+template <int N, int B>
+__global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid,
+                                                       const float2* __restrict__ tw,
+                                                       const float2* __restrict__ rot_y) {
+  __shared__ float2 lds[B * N];
+  const int col0 = blockIdx.x * B;
+  for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+    const int p = idx / B, t = idx % B;
+    lds[idx] = grid[(long)p * N + col0 + t];
+  }
+  __syncthreads();
+  fft_lds<N, B, true>(lds, tw);
+  for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+    const int p = idx / B, t = idx % B;
+    grid[(long)p * N + col0 + t] = cmul(lds[idx], rot_y[p]);
+  }
+}
+
+template <int N>
+gvm_status launch_fft(gvm_ctx* ctx, const float* I, const float* pb, float nudiv,
+                      float floor_v) {
+  constexpr int BR = (4096 / N) > 0 ? (4096 / N) : 1;
+  constexpr int BC0 = 8192 / N;
+  constexpr int BC = BC0 < 2 ? 2 : (BC0 > N ? N : BC0);
+  hipLaunchKernelGGL((fft_rows_kernel<N, BR>), dim3(N / BR), dim3(256), 0, ctx->stream, I, pb,
+                     ctx->g.fg_scale, nudiv, floor_v, ctx->twiddle, ctx->rot_x, ctx->grid);
+  GVM_CHECK_LAUNCH();
+  hipLaunchKernelGGL((fft_cols_kernel<N, BC>), dim3(N / BC), dim3(256), 0, ctx->stream,
+                     ctx->grid, ctx->twiddle, ctx->rot_y);
+  GVM_CHECK_LAUNCH();
+  return GVM_OK;
+}
+
+gvm_status model_grid(gvm_ctx* ctx, const float* I, const Slice& s) {
+  const float nudiv = s.freq / ctx->g.nu_0;
+  const float floor_v = (float)(-1.0 * (double)ctx->g.eta * (double)ctx->g.minpix);
+  switch (ctx->g.N) {
+    case 64: return launch_fft<64>(ctx, I, s.pb, nudiv, floor_v);
+    case 128: return launch_fft<128>(ctx, I, s.pb, nudiv, floor_v);
+    case 256: return launch_fft<256>(ctx, I, s.pb, nudiv, floor_v);
+    case 512: return launch_fft<512>(ctx, I, s.pb, nudiv, floor_v);
+    case 1024: return launch_fft<1024>(ctx, I, s.pb, nudiv, floor_v);
+    case 2048: return launch_fft<2048>(ctx, I, s.pb, nudiv, floor_v);
+    case 4096: return launch_fft<4096>(ctx, I, s.pb, nudiv, floor_v);
+  }
+  set_error("model_grid: unsupported N");
+  return GVM_ERR_ARG;
+}
+
+// ---------------------------------------------------------------------------
+// vis_mod + residual + chi2Vector + reduction:
+// src/functions.cu:2557-2609, 2663-2671, 2867-2876.  Bytes per visibility:
+// cell 4 + frac 8 + Vo 8 + weight 4 read, Vr 8 written.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void degrid_chi2_kernel(
+    const float2* __restrict__ grid, int N, const int* __restrict__ cell,
+    const float2* __restrict__ frac, const float2* __restrict__ vo,
+    const float* __restrict__ weight, float2* __restrict__ vr, long nvis,
+    double* __restrict__ partials, unsigned int* __restrict__ cmax) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  float cm = 0.0f;
+  const int mask = N - 1;
+  for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < nvis;
+       v += (long)gridDim.x * blockDim.x) {
+    const int c = cell[v];
+    float2 m = make_float2(0.f, 0.f);
+    if (c >= 0) {
+      const int i1 = c & 0xffff, j1 = c >> 16;
+      const int i2 = (i1 + 1) & mask, j2 = (j1 + 1) & mask;
+      const float2 f = frac[v];
+      const float2 v11 = grid[(long)N * j1 + i1];
+      const float2 v12 = grid[(long)N * j2 + i1];
+      const float2 v21 = grid[(long)N * j1 + i2];
+      const float2 v22 = grid[(long)N * j2 + i2];
+      const float w11 = (1.0f - f.x) * (1.0f - f.y);
+      const float w12 = (1.0f - f.x) * f.y;
+      const float w21 = f.x * (1.0f - f.y);
+      const float w22 = f.x * f.y;
+      m.x = w11 * v11.x + w12 * v12.x + w21 * v21.x + w22 * v22.x;
+      m.y = w11 * v11.y + w12 * v12.y + w21 * v21.y + w22 * v22.y;
+    }
+    const float2 o = vo[v];
+    const float2 r = make_float2(o.x - m.x, o.y - m.y);
+    vr[v] = r;
+    const float w = weight[v];
+    const float a2 = r.x * r.x + r.y * r.y;
+    acc += (double)(w * a2);
+    cm = fmaxf(cm, w * sqrtf(a2));
+  }
+  const double tot = block_sum(acc, sh);
+  __syncthreads();
+  const double mx = block_max((double)cm, sh);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = tot;
+    atomicMax(cmax, __float_as_uint((float)mx));
+  }
+}
+
+gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize, double* out) {
+  const gvm_geometry& g = ctx->g;
+  const long n = (long)g.M * g.N;
+  const int nsl = (int)ctx->slices.size();
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  const float floor_v = (float)(-1.0 * (double)g.eta * (double)g.minpix);
+  hipLaunchKernelGGL(clip_kernel, dim3(reduce_grid(n)), dim3(256), 0, ctx->stream, I, ctx->noise,
+                     n, g.noise_cut, floor_v, g.eta > 0.0f ? 1 : 0, g.threshold, flag_opt);
+  GVM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(zero_u32, dim3((nsl + 255) / 256), dim3(256), 0, ctx->stream, ctx->cmax,
+                     nsl);
+  GVM_CHECK_LAUNCH();
+  for (int si = 0; si < nsl; si++) {
+    const Slice& s = ctx->slices[si];
+    gvm_status st = model_grid(ctx, I, s);
+    if (st) return st;
+    hipLaunchKernelGGL(degrid_chi2_kernel, dim3(kReduceBlocks), dim3(256), 0, ctx->stream,
+                       ctx->grid, g.N, s.cell, s.frac, s.vo, s.weight, s.vr, s.nvis,
+                       ctx->partials + (long)si * kReduceBlocks, ctx->cmax + si);
+    GVM_CHECK_LAUNCH();
+  }
+  gvm_status st = finish_reductions(ctx, nsl, kReduceBlocks);
+  if (st) return st;
+  std::vector<double> sums(nsl);
+  st = read_scalars(ctx, nsl, sums.data());
+  if (st) return st;
+  double total = 0.0;
+  for (int si = 0; si < nsl; si++) {
+    double r = sums[si];
+    if (normalize) r /= (double)ctx->slices[si].nvis;
+    total += r;
+  }
+  ctx->have_residuals = true;
+  *out = 0.5 * total;
+  return GVM_OK;
+}
+
+}  // namespace gvm

@@ -1,0 +1,143 @@
+// Internal declarations shared by the HIP translation units of libgvm_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gvm_hip.h"
+
+namespace gvm {
+
+// Thread-local text of the last error (gvm_last_error).
+void set_error(const std::string& msg);
+const char* last_error_cstr();
+
+#define GVM_HIP_TRY(expr)                                                   \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      ::gvm::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+      return e_ == hipErrorOutOfMemory ? GVM_ERR_OOM : GVM_ERR_HIP;         \
+    }                                                                       \
+  } while (0)
+
+#define GVM_CHECK_LAUNCH() GVM_HIP_TRY(hipGetLastError())
+
+#define GVM_REQUIRE(cond, msg)     \
+  do {                             \
+    if (!(cond)) {                 \
+      ::gvm::set_error(msg);       \
+      return GVM_ERR_ARG;          \
+    }                              \
+  } while (0)
+
+// One channel x stokes slice of visibilities, resident in HBM.
+struct Slice {
+  float freq = 0.f;
+  long nvis = 0;
+  // static per-visibility data (set-up)
+  int* cell = nullptr;       // i1 | (j1 << 16) of the bilinear stencil, -1 off-grid
+  float2* frac = nullptr;    // (du, dv) bilinear fractions
+  float2* vo = nullptr;      // observed visibilities
+  float* weight = nullptr;   // weights (0 off-grid)
+  float4* phs = nullptr;     // (a_hi, a_lo, b_hi, b_lo): u dx and v dy split for exact turns
+  float* wcoord = nullptr;   // w in wavelengths
+  float* pb = nullptr;       // primary beam at this frequency, float[M*N]
+  // per-evaluation data
+  float2* vr = nullptr;      // residuals Vo - Vm
+};
+
+constexpr int kReduceBlocks = 1024;
+constexpr int kReduceThreads = 256;
+constexpr int kMaxSlots = 64;
+
+}  // namespace gvm
+
+struct gvm_ctx {
+  gvm_geometry g{};
+  int device = 0;
+  int cu_count = 0;
+  hipStream_t stream = nullptr;
+  // image-space tables
+  float* noise = nullptr;       // float[M*N]
+  float2* grid = nullptr;       // uv grid, complex float[N*M]
+  float2* twiddle = nullptr;    // e^{+2 pi i k / N}, k < N
+  float2* rot_x = nullptr;      // phase rotation factor per column frequency
+  float2* rot_y = nullptr;      // phase rotation factor per row frequency
+  float2* xtab = nullptr;       // (j - int(x0_phs), sqrt(1 - x^2) - 1) per column
+  float2* ytab = nullptr;       // (i - int(y0_phs), sqrt(1 - y^2) - 1) per row
+  // reductions
+  double* partials = nullptr;   // kMaxSlots * kReduceBlocks doubles
+  double* dscalars = nullptr;   // kMaxSlots doubles (device)
+  double* hscalars = nullptr;   // kMaxSlots doubles (pinned host)
+  unsigned int* cmax = nullptr; // per-slice max |w Vr| (float bits), device
+  // adjoint workspace
+  float* adj_part = nullptr;    // split-K partial tiles
+  size_t adj_part_elems = 0;
+  int adjoint_mode = GVM_ADJOINT_F16X3;
+  std::vector<gvm::Slice> slices;
+  bool have_residuals = false;
+  // timing of the adjoint GEMM
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  double adj_ms_total = 0.0;
+  long adj_count = 0;
+};
+
+namespace gvm {
+
+// Launchers implemented in the .hip files.
+gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize,
+                        double* out);
+gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt,
+                   int normalize);
+// Block-partial reduction helpers: reduce `nblocks` partials of slot s into
+// dscalars[s] (device), in a fixed order.
+gvm_status finish_reductions(gvm_ctx* ctx, int nslots, int nblocks);
+gvm_status finish_reductions_mode(gvm_ctx* ctx, int nslots, int nblocks, int is_max);
+gvm_status read_scalars(gvm_ctx* ctx, int nslots, double* out);
+int reduce_grid(long n);
+
+// Wave-level (64 lanes) and block-level sums in double.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sums v over a block of kReduceThreads threads; the result is valid in
+// thread 0.  `sh` must hold kReduceThreads / 64 doubles.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
+  }
+  return r;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    r = sh[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, sh[w]);
+  }
+  return r;
+}
+
+}  // namespace gvm

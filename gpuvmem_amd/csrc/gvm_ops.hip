@@ -1,0 +1,411 @@
+// Prior values/gradients and the optimiser's vector operations on gfx950.
+//
+// Priors: reference src/functions.cu:3036-3112 (entropy), 3353-3441 (TSV),
+// 3443-3545 (Laplacian).  Values are FP32 per pixel, summed in FP64 per
+// block and then in a fixed order (no float tree, no host loop); gradients
+// are written in one pass.  Optimiser vector kernels: src/functions.cu
+// :2729-2865, 3564-3687; the reductions that the reference runs as a float
+// reduction + D2H per call (deviceReduce / deviceMaxReduce, :605-879) are
+// FP64 here.
+#include <hip/hip_runtime.h>
+
+#include "gvm_internal.h"
+
+namespace gvm {
+
+// ---------------------------------------------------------------------------
+// priors
+// ---------------------------------------------------------------------------
+enum { PRIOR_ENTROPY = 0, PRIOR_TSV = 1, PRIOR_LAPLACIAN = 2 };
+
+// calculateS / calculateTSV / calculateL for pixel (i, j) of image `im`.
+__device__ __forceinline__ float prior_term(int kind, const float* __restrict__ im, int N, int i,
+                                            int j, float G, float eta) {
+  const long k = (long)N * i + j;
+  const float c = im[k];
+  if (kind == PRIOR_ENTROPY) return c * logf((c / G) + (eta + 1.0f));
+  if (kind == PRIOR_TSV) {
+    if (i < N - 1 && j < N - 1) {
+      const float dx = c - im[k + 1], dy = c - im[k + N];
+      return dx * dx + dy * dy;
+    }
+    return c;
+  }
+  if (i > 0 && i < N - 1 && j > 0 && j < N - 1) {
+    const float Dx = im[k - 1] - 2.0f * c + im[k + 1];
+    const float Dy = im[k - N] - 2.0f * c + im[k + N];
+    return 0.5f * (Dx + Dy) * (Dx + Dy);
+  }
+  return c;
+}
+
+// calculateDS / calculateDTSV / calculateDL (without lambda).
+__device__ __forceinline__ float prior_grad(int kind, const float* __restrict__ im, int N, int i,
+                                            int j, float G, float eta) {
+  const long k = (long)N * i + j;
+  const float c = im[k];
+  if (kind == PRIOR_ENTROPY)
+    return logf((c / G) + (eta + 1.0f)) + 1.0f / (1.0f + (((eta + 1.0f) * G) / c));
+  if (kind == PRIOR_TSV) {
+    if (i > 0 && i < N - 1 && j > 0 && j < N - 1)
+      return 8.0f * c - 2.0f * (im[k - N] + im[k - 1] + im[k + N] + im[k + 1]);
+    return c;
+  }
+  if (i > 1 && i < N - 2 && j > 1 && j < N - 2) {
+    const float d = im[k + N], u = im[k - N], r = im[k + 1], l = im[k - 1];
+    const float corners = im[k + N - 1] + im[k + N + 1] + im[k - N - 1] + im[k - N + 1];
+    const float far = im[k + 2 * N] + im[k + 2] + im[k - 2 * N] + im[k - 2];
+    // the reference's stencil: -8 (d - r - u - l)
+    return 20.0f * c - 8.0f * (d - r - u - l) + 2.0f * corners + far;
+  }
+  return 0.0f;
+}
+
+__global__ __launch_bounds__(256) void prior_value_kernel(int kind, const float* __restrict__ im,
+                                                          const float* __restrict__ noise,
+                                                          float noise_cut, int N, float G,
+                                                          float eta, double* __restrict__ partials) {
+  __shared__ double sh[4];
+  const long n = (long)N * N;
+  double acc = 0.0;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x) {
+    if (noise[k] < noise_cut) acc += (double)prior_term(kind, im, N, (int)(k / N), (int)(k % N), G, eta);
+  }
+  const double tot = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void prior_grad_kernel(int kind, const float* __restrict__ im,
+                                                         const float* __restrict__ noise,
+                                                         float noise_cut, int N, float G, float eta,
+                                                         float lambda, float* __restrict__ dgi) {
+  const long n = (long)N * N;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x) {
+    float d = 0.0f;
+    if (noise[k] < noise_cut) d = prior_grad(kind, im, N, (int)(k / N), (int)(k % N), G, eta);
+    dgi[k] = d * lambda;
+  }
+}
+
+static gvm_status prior_value(gvm_ctx* ctx, int kind, const float* I, int index, float G,
+                              float eta, double* out) {
+  GVM_REQUIRE(ctx && I && out, "prior value: null argument");
+  GVM_REQUIRE(index >= 0 && index < ctx->g.image_count, "prior value: bad image index");
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  const int N = ctx->g.N;
+  const long n = (long)N * N;
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(prior_value_kernel, dim3(nb), dim3(256), 0, ctx->stream, kind, I + n * index,
+                     ctx->noise, ctx->g.noise_cut, N, G, eta, ctx->partials);
+  GVM_CHECK_LAUNCH();
+  gvm_status st = finish_reductions(ctx, 1, nb);
+  if (st) return st;
+  return read_scalars(ctx, 1, out);
+}
+
+static gvm_status prior_grad(gvm_ctx* ctx, int kind, const float* I, float* dgi, int index,
+                             float G, float eta, float lambda) {
+  GVM_REQUIRE(ctx && I && dgi, "prior gradient: null argument");
+  GVM_REQUIRE(index >= 0 && index < ctx->g.image_count, "prior gradient: bad image index");
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  const int N = ctx->g.N;
+  const long n = (long)N * N;
+  hipLaunchKernelGGL(prior_grad_kernel, dim3(reduce_grid(n)), dim3(256), 0, ctx->stream, kind,
+                     I + n * index, ctx->noise, ctx->g.noise_cut, N, G, eta, lambda, dgi);
+  GVM_CHECK_LAUNCH();
+  return GVM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+#define GRID_LOOP(k, n) \
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < (n); k += (long)gridDim.x * blockDim.x)
+
+__global__ void add_kernel(float* __restrict__ y, const float* __restrict__ x, long n) {
+  GRID_LOOP(k, n) y[k] += x[k];
+}
+__global__ void fill_kernel(float* __restrict__ y, float v, long n) {
+  GRID_LOOP(k, n) y[k] = v;
+}
+__global__ void copy_kernel(float* __restrict__ y, const float* __restrict__ x, long n) {
+  GRID_LOOP(k, n) y[k] = x[k];
+}
+__global__ void scale_kernel(float* y, const float* x, float a, long n) {
+  GRID_LOOP(k, n) y[k] = x[k] * a;
+}
+__global__ void axpy_kernel(float* __restrict__ y, const float* __restrict__ x, float a, long n) {
+  GRID_LOOP(k, n) y[k] += a * x[k];
+}
+__global__ void search_direction_kernel(float* __restrict__ g, float* __restrict__ xi,
+                                        float* __restrict__ h, long n) {
+  GRID_LOOP(k, n) {
+    const float v = -xi[k];
+    g[k] = v;
+    xi[k] = v;
+    h[k] = v;
+  }
+}
+__global__ void new_xi_kernel(float* __restrict__ g, float* __restrict__ xi, float* __restrict__ h,
+                              float gam, long n) {
+  GRID_LOOP(k, n) {
+    const float gv = -xi[k];
+    g[k] = gv;
+    const float v = gv + gam * h[k];
+    xi[k] = v;
+    h[k] = v;
+  }
+}
+__global__ void evaluate_xt_kernel(float* __restrict__ xt, const float* __restrict__ pcom,
+                                   const float* __restrict__ xicom, float x, int positivity,
+                                   float floor_v, long n) {
+  GRID_LOOP(k, n) {
+    const float v = pcom[k] + x * xicom[k];
+    xt[k] = (positivity && !(v > floor_v)) ? floor_v : v;
+  }
+}
+__global__ void new_p_kernel(float* __restrict__ p, float* __restrict__ xi, float xmin,
+                             int positivity, float floor_v, long n) {
+  GRID_LOOP(k, n) {
+    const float d = xi[k] * xmin;
+    if (!positivity || p[k] + d > floor_v) {
+      xi[k] = d;
+      p[k] += d;
+    } else {
+      xi[k] = 0.0f;
+      p[k] = floor_v;
+    }
+  }
+}
+__global__ void s_and_y_kernel(float* __restrict__ y, float* __restrict__ s,
+                               const float* __restrict__ p, const float* __restrict__ xi,
+                               const float* __restrict__ p_old, const float* __restrict__ xi_old,
+                               long n) {
+  GRID_LOOP(k, n) {
+    y[k] = xi[k] - (-1.0f * xi_old[k]);
+    s[k] = p[k] - p_old[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void dot_kernel(const float* __restrict__ a,
+                                                  const float* __restrict__ b, long n,
+                                                  double* __restrict__ partials) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  GRID_LOOP(k, n) acc += (double)a[k] * (double)b[k];
+  const double tot = block_sum(acc, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void gg_dgg_kernel(const float* __restrict__ xi,
+                                                     const float* __restrict__ g, long n,
+                                                     double* __restrict__ partials) {
+  __shared__ double sh[4];
+  double gg = 0.0, dgg = 0.0;
+  GRID_LOOP(k, n) {
+    const double gv = g[k], xv = xi[k];
+    gg += gv * gv;
+    dgg += (xv + gv) * xv;
+  }
+  const double t1 = block_sum(gg, sh);
+  __syncthreads();
+  const double t2 = block_sum(dgg, sh);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = t1;
+    partials[kReduceBlocks + blockIdx.x] = t2;
+  }
+}
+
+__global__ __launch_bounds__(256) void absmax_kernel(const float* __restrict__ x, long n,
+                                                     double* __restrict__ partials) {
+  __shared__ double sh[4];
+  double m = 0.0;
+  GRID_LOOP(k, n) m = fmax(m, (double)fabsf(x[k]));
+  const double r = block_max(m, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void grad_condition_kernel(const float* __restrict__ xi,
+                                                             const float* __restrict__ p, double den,
+                                                             long n, double* __restrict__ partials) {
+  __shared__ double sh[4];
+  double m = 0.0;
+  GRID_LOOP(k, n) m = fmax(m, (double)fabsf(xi[k]) * fmax((double)fabsf(p[k]), 1.0) / den);
+  const double r = block_max(m, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+#undef GRID_LOOP
+
+}  // namespace gvm
+
+using namespace gvm;
+
+#define LAUNCH_1D(kernel, n, ...)                                                        \
+  do {                                                                                   \
+    hipLaunchKernelGGL(kernel, dim3(reduce_grid(n)), dim3(256), 0, ctx->stream, __VA_ARGS__); \
+    GVM_CHECK_LAUNCH();                                                                  \
+  } while (0)
+
+static long img_elems(const gvm_ctx* ctx) { return (long)ctx->g.M * ctx->g.N; }
+
+extern "C" gvm_status gvm_entropy_value(gvm_ctx* ctx, const float* I, int index, float prior,
+                                        float eta, double* out) {
+  return prior_value(ctx, PRIOR_ENTROPY, I, index, prior, eta, out);
+}
+extern "C" gvm_status gvm_entropy_grad(gvm_ctx* ctx, const float* I, float* dgi, int index,
+                                       float prior, float eta, float lambda) {
+  return prior_grad(ctx, PRIOR_ENTROPY, I, dgi, index, prior, eta, lambda);
+}
+extern "C" gvm_status gvm_tsv_value(gvm_ctx* ctx, const float* I, int index, double* out) {
+  return prior_value(ctx, PRIOR_TSV, I, index, 0.f, 0.f, out);
+}
+extern "C" gvm_status gvm_tsv_grad(gvm_ctx* ctx, const float* I, float* dgi, int index,
+                                   float lambda) {
+  return prior_grad(ctx, PRIOR_TSV, I, dgi, index, 0.f, 0.f, lambda);
+}
+extern "C" gvm_status gvm_laplacian_value(gvm_ctx* ctx, const float* I, int index, double* out) {
+  return prior_value(ctx, PRIOR_LAPLACIAN, I, index, 0.f, 0.f, out);
+}
+extern "C" gvm_status gvm_laplacian_grad(gvm_ctx* ctx, const float* I, float* dgi, int index,
+                                         float lambda) {
+  return prior_grad(ctx, PRIOR_LAPLACIAN, I, dgi, index, 0.f, 0.f, lambda);
+}
+
+extern "C" gvm_status gvm_add_to_dphi(gvm_ctx* ctx, float* dphi, const float* dgi, int index) {
+  GVM_REQUIRE(ctx && dphi && dgi, "gvm_add_to_dphi: null argument");
+  GVM_REQUIRE(index >= 0 && index < ctx->g.image_count, "gvm_add_to_dphi: bad image index");
+  const long n = img_elems(ctx);
+  LAUNCH_1D(add_kernel, n, dphi + n * index, dgi, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_vec_zero(gvm_ctx* ctx, float* x, long n) {
+  GVM_REQUIRE(ctx && x && n >= 0, "gvm_vec_zero: bad argument");
+  if (n == 0) return GVM_OK;
+  LAUNCH_1D(fill_kernel, n, x, 0.0f, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_vec_copy(gvm_ctx* ctx, float* dst, const float* src, long n) {
+  GVM_REQUIRE(ctx && dst && src && n >= 0, "gvm_vec_copy: bad argument");
+  if (n == 0 || dst == src) return GVM_OK;
+  LAUNCH_1D(copy_kernel, n, dst, src, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_cg_search_direction(gvm_ctx* ctx, float* g, float* xi, float* h,
+                                              int image) {
+  GVM_REQUIRE(ctx && g && xi && h, "gvm_cg_search_direction: null argument");
+  GVM_REQUIRE(image >= 0 && image < ctx->g.image_count, "gvm_cg_search_direction: bad image");
+  const long n = img_elems(ctx), o = n * image;
+  LAUNCH_1D(search_direction_kernel, n, g + o, xi + o, h + o, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_cg_new_xi(gvm_ctx* ctx, float* g, float* xi, float* h, double gam,
+                                    int image) {
+  GVM_REQUIRE(ctx && g && xi && h, "gvm_cg_new_xi: null argument");
+  GVM_REQUIRE(image >= 0 && image < ctx->g.image_count, "gvm_cg_new_xi: bad image");
+  const long n = img_elems(ctx), o = n * image;
+  LAUNCH_1D(new_xi_kernel, n, g + o, xi + o, h + o, (float)gam, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_cg_grad_condition(gvm_ctx* ctx, const float* xi, const float* p,
+                                            double den, int nimg, double* gmax) {
+  GVM_REQUIRE(ctx && xi && p && gmax && nimg > 0, "gvm_cg_grad_condition: bad argument");
+  const long n = img_elems(ctx) * nimg;
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(grad_condition_kernel, dim3(nb), dim3(256), 0, ctx->stream, xi, p, den, n,
+                     ctx->partials);
+  GVM_CHECK_LAUNCH();
+  gvm_status st = finish_reductions_mode(ctx, 1, nb, 1);
+  if (st) return st;
+  return read_scalars(ctx, 1, gmax);
+}
+
+extern "C" gvm_status gvm_cg_gg_dgg(gvm_ctx* ctx, const float* xi, const float* g, int nimg,
+                                    double* gg, double* dgg) {
+  GVM_REQUIRE(ctx && xi && g && gg && dgg && nimg > 0, "gvm_cg_gg_dgg: bad argument");
+  const long n = img_elems(ctx) * nimg;
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(gg_dgg_kernel, dim3(nb), dim3(256), 0, ctx->stream, xi, g, n, ctx->partials);
+  GVM_CHECK_LAUNCH();
+  gvm_status st = finish_reductions(ctx, 2, nb);
+  if (st) return st;
+  double v[2];
+  st = read_scalars(ctx, 2, v);
+  if (st) return st;
+  *gg = v[0];
+  *dgg = v[1];
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_evaluate_xt(gvm_ctx* ctx, float* xt, const float* pcom,
+                                      const float* xicom, double x, int image, int positivity,
+                                      float minpix, float eta) {
+  GVM_REQUIRE(ctx && xt && pcom && xicom, "gvm_evaluate_xt: null argument");
+  GVM_REQUIRE(image >= 0 && image < ctx->g.image_count, "gvm_evaluate_xt: bad image");
+  const long n = img_elems(ctx), o = n * image;
+  const float floor_v = (float)(-1.0 * (double)eta * (double)minpix);
+  LAUNCH_1D(evaluate_xt_kernel, n, xt + o, pcom + o, xicom + o, (float)x, positivity, floor_v, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_new_p(gvm_ctx* ctx, float* p, float* xi, double xmin, int image,
+                                int positivity, float minpix, float eta) {
+  GVM_REQUIRE(ctx && p && xi, "gvm_new_p: null argument");
+  GVM_REQUIRE(image >= 0 && image < ctx->g.image_count, "gvm_new_p: bad image");
+  const long n = img_elems(ctx), o = n * image;
+  const float floor_v = (float)(-1.0 * (double)eta * (double)minpix);
+  LAUNCH_1D(new_p_kernel, n, p + o, xi + o, (float)xmin, positivity, floor_v, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_vec_absmax(gvm_ctx* ctx, const float* x, long n, double* out) {
+  GVM_REQUIRE(ctx && x && out && n > 0, "gvm_vec_absmax: bad argument");
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(absmax_kernel, dim3(nb), dim3(256), 0, ctx->stream, x, n, ctx->partials);
+  GVM_CHECK_LAUNCH();
+  gvm_status st = finish_reductions_mode(ctx, 1, nb, 1);
+  if (st) return st;
+  return read_scalars(ctx, 1, out);
+}
+
+extern "C" gvm_status gvm_vec_scale(gvm_ctx* ctx, float* dst, const float* src, double scale,
+                                    long n) {
+  GVM_REQUIRE(ctx && dst && src && n >= 0, "gvm_vec_scale: bad argument");
+  if (n == 0) return GVM_OK;
+  LAUNCH_1D(scale_kernel, n, dst, src, (float)scale, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_vec_dot(gvm_ctx* ctx, const float* a, const float* b, long n,
+                                  double* out) {
+  GVM_REQUIRE(ctx && a && b && out && n > 0, "gvm_vec_dot: bad argument");
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, ctx->stream, a, b, n, ctx->partials);
+  GVM_CHECK_LAUNCH();
+  gvm_status st = finish_reductions(ctx, 1, nb);
+  if (st) return st;
+  return read_scalars(ctx, 1, out);
+}
+
+extern "C" gvm_status gvm_vec_axpy(gvm_ctx* ctx, float* y, const float* x, double alpha, long n) {
+  GVM_REQUIRE(ctx && y && x && n >= 0, "gvm_vec_axpy: bad argument");
+  if (n == 0) return GVM_OK;
+  LAUNCH_1D(axpy_kernel, n, y, x, (float)alpha, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_lbfgs_s_and_y(gvm_ctx* ctx, float* y, float* s, const float* p,
+                                        const float* xi, const float* p_old, const float* xi_old,
+                                        long n) {
+  GVM_REQUIRE(ctx && y && s && p && xi && p_old && xi_old && n > 0,
+              "gvm_lbfgs_s_and_y: bad argument");
+  LAUNCH_1D(s_and_y_kernel, n, y, s, p, xi, p_old, xi_old, n);
+  return GVM_OK;
+}

@@ -1,0 +1,279 @@
+// Fi base, ObjectiveFunction and the objective terms (Chi2, Entropy,
+// TotalSquaredVariation, Laplacian) over the libgvm_hip C ABI.
+#include <cstdio>
+#include <string>
+
+#include "gvm/factory.h"
+#include "gvm/framework.h"
+#include "gvm/objectivefunction.h"
+#include "gvm/terms.h"
+
+namespace gvm {
+
+Globals& globals() {
+  static Globals g;
+  return g;
+}
+
+void check(gvm_status st, const char* what) {
+  if (st != GVM_OK)
+    throw Error(std::string(what) + " failed (status " + std::to_string(st) + "): " +
+                    gvm_last_error(),
+                st);
+}
+
+static float* dev_alloc_zero(long count) {
+  void* p = nullptr;
+  check(gvm_malloc(&p, sizeof(float) * (size_t)count), "gvm_malloc");
+  check(gvm_vec_zero(globals().ctx, (float*)p, count), "gvm_vec_zero");
+  return (float*)p;
+}
+
+static gvm_ctx* ctx() {
+  gvm_ctx* c = globals().ctx;
+  if (!c) throw Error("no current gvm context (gvm::globals().ctx)", GVM_ERR_STATE);
+  return c;
+}
+
+}  // namespace gvm
+
+using gvm::check;
+using gvm::globals;
+
+// ---------------------------------------------------------------------------
+// Fi: include/classes/fi.cuh:13-104
+// ---------------------------------------------------------------------------
+Fi::Fi() : name("default") {}
+
+Fi::~Fi() {
+  gvm_free(device_S);
+  gvm_free(device_DS);
+}
+
+void Fi::configure(int penalizatorIndex, int imageIndex, int imageToAdd, bool normalize) {
+  const auto& g = globals();
+  this->imageIndex = imageIndex;
+  this->imageToAdd = imageToAdd;
+  this->normalize = normalize;
+  if (imageIndex > g.image_count - 1 || imageToAdd > g.image_count - 1)
+    throw gvm::Error("There is no image for the provided index " + name, GVM_ERR_ARG);
+  if (penalizatorIndex != -1) {
+    if (penalizatorIndex < 0)
+      throw gvm::Error("invalid index for penalizator (" + name + ")", GVM_ERR_ARG);
+    else if (penalizatorIndex > (int)g.penalizators.size() - 1)
+      this->penalization_factor = 0.0f;
+    else
+      this->penalization_factor = g.penalizators[penalizatorIndex];
+  }
+  gvm_free(device_S);
+  gvm_free(device_DS);
+  device_S = gvm::dev_alloc_zero(g.M * g.N);
+  device_DS = gvm::dev_alloc_zero(g.M * g.N);
+}
+
+// ---------------------------------------------------------------------------
+// ObjectiveFunction: include/classes/objectivefunction.cuh:6-103
+// ---------------------------------------------------------------------------
+ObjectiveFunction::~ObjectiveFunction() { gvm_free(dphi); }
+
+void ObjectiveFunction::addFi(Fi* fi) {
+  if (fi->getPenalizationFactor()) {
+    fis.push_back(fi);
+    fi_values.push_back(0.0f);
+  }
+}
+
+double ObjectiveFunction::calcFunctionD(float* p) {
+  double value = 0.0;
+  n_fcalls++;
+  for (size_t k = 0; k < fis.size(); k++) {
+    value += fis[k]->calcFiD(p);
+    fi_values[k] = fis[k]->get_fivalue();
+  }
+  return value;
+}
+
+float ObjectiveFunction::calcFunction(float* p) { return (float)calcFunctionD(p); }
+
+void ObjectiveFunction::calcGradient(float* p, float* xi, int iter) {
+  n_gcalls++;
+  restartDPhi();
+  for (Fi* fi : fis) {
+    fi->setIteration(iter);
+    fi->calcGi(p, xi);
+    fi->addToDphi(dphi);
+  }
+  copyDphiToXi(xi);
+}
+
+void ObjectiveFunction::restartDPhi() {
+  for (Fi* fi : fis) fi->restartDGi();
+  check(gvm_vec_zero(gvm::ctx(), dphi, M * N * image_count), "gvm_vec_zero(dphi)");
+}
+
+void ObjectiveFunction::copyDphiToXi(float* xi) {
+  check(gvm_vec_copy(gvm::ctx(), xi, dphi, M * N * image_count), "gvm_vec_copy(xi)");
+}
+
+Fi* ObjectiveFunction::getFiByName(std::string fi_name) {
+  for (Fi* fi : fis)
+    if (fi->getName() == fi_name) return fi;
+  return nullptr;
+}
+
+void ObjectiveFunction::configure(long N, long M, int I) {
+  setN(N);
+  setM(M);
+  setImageCount(I);
+  gvm_free(dphi);
+  dphi = gvm::dev_alloc_zero(M * N * I);
+}
+
+namespace {
+ObjectiveFunction* CreateObjectiveFunction() { return new ObjectiveFunction; }
+const bool RegisteredObjectiveFunction =
+    registerCreationFunction<ObjectiveFunction, std::string>("ObjectiveFunction",
+                                                             CreateObjectiveFunction);
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Chi2: src/chi2.cu:13-92
+// ---------------------------------------------------------------------------
+Chi2::Chi2() { this->name = "Chi2"; }
+
+Chi2::~Chi2() { gvm_free(result_dchi2); }
+
+void Chi2::configure(int penalizatorIndex, int imageIndex, int imageToAdd, bool normalize) {
+  const auto& g = globals();
+  this->imageIndex = imageIndex;
+  this->imageToAdd = imageToAdd;
+  this->normalize = normalize;
+  if (penalizatorIndex != -1) {
+    if (penalizatorIndex > (int)g.penalizators.size() - 1 || penalizatorIndex < 0)
+      throw gvm::Error("invalid index for penalizator (" + name + ")", GVM_ERR_ARG);
+    this->penalization_factor = g.penalizators[penalizatorIndex];
+  }
+  gvm_free(result_dchi2);
+  result_dchi2 = gvm::dev_alloc_zero(g.M * g.N * g.image_count);
+}
+
+double Chi2::calcFiD(float* p) {
+  double v = 0.0;
+  check(gvm_chi2(gvm::ctx(), p, globals().flag_opt, normalize ? 1 : 0, &v), "gvm_chi2");
+  set_fivalue(v);
+  return (double)penalization_factor * v;
+}
+
+float Chi2::calcFi(float* p) { return (float)calcFiD(p); }
+
+void Chi2::calcGi(float* p, float* xi) {
+  check(gvm_dchi2(gvm::ctx(), p, result_dchi2, globals().flag_opt, normalize ? 1 : 0),
+        "gvm_dchi2");
+}
+
+void Chi2::restartDGi() {
+  const auto& g = globals();
+  check(gvm_vec_zero(gvm::ctx(), result_dchi2, g.M * g.N * g.image_count), "gvm_vec_zero");
+}
+
+// src/chi2.cu:60-70: with more than one image dphi is cleared and replaced by
+// the chi2 gradient, so Chi2 has to be the first term.
+void Chi2::addToDphi(float* device_dphi) {
+  const auto& g = globals();
+  if (g.image_count == 1) check(gvm_add_to_dphi(gvm::ctx(), device_dphi, result_dchi2, 0), "AddToDPhi");
+  if (g.image_count > 1) {
+    check(gvm_vec_zero(gvm::ctx(), device_dphi, g.M * g.N * g.image_count), "gvm_vec_zero");
+    check(gvm_vec_copy(gvm::ctx(), device_dphi, result_dchi2, g.M * g.N * g.image_count),
+          "gvm_vec_copy");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Priors.  Host launchers src/functions.cu:4722-4964: value and gradient are
+// zero at iteration 0 or with lambda 0; the gradient is computed only when
+// flag_opt % 2 == imageIndex.
+// ---------------------------------------------------------------------------
+static bool prior_active(int iteration, float lambda) { return iteration > 0 && lambda != 0.0f; }
+
+Entropy::Entropy() : prior_value(1.0f), eta(-1.0f) { this->name = "Entropy"; }
+Entropy::Entropy(float prior_value) : prior_value(prior_value), eta(-1.0f) { this->name = "Entropy"; }
+Entropy::Entropy(float prior_value, float eta) : prior_value(prior_value), eta(eta) {
+  this->name = "Entropy";
+}
+
+double Entropy::calcFiD(float* p) {
+  double v = 0.0;
+  if (prior_active(iteration, penalization_factor))
+    check(gvm_entropy_value(gvm::ctx(), p, imageIndex, prior_value, eta, &v), "gvm_entropy_value");
+  set_fivalue(v);
+  return (double)penalization_factor * v;
+}
+float Entropy::calcFi(float* p) { return (float)calcFiD(p); }
+
+void Entropy::calcGi(float* p, float* xi) {
+  if (prior_active(iteration, penalization_factor) && globals().flag_opt % 2 == imageIndex)
+    check(gvm_entropy_grad(gvm::ctx(), p, device_DS, imageIndex, prior_value, eta,
+                           penalization_factor),
+          "gvm_entropy_grad");
+}
+void Entropy::restartDGi() {
+  check(gvm_vec_zero(gvm::ctx(), device_DS, globals().M * globals().N), "gvm_vec_zero");
+}
+void Entropy::addToDphi(float* device_dphi) {
+  check(gvm_add_to_dphi(gvm::ctx(), device_dphi, device_DS, imageToAdd), "AddToDPhi");
+}
+
+TotalSquaredVariationP::TotalSquaredVariationP() { this->name = "Total Squared Variation"; }
+double TotalSquaredVariationP::calcFiD(float* p) {
+  double v = 0.0;
+  if (prior_active(iteration, penalization_factor))
+    check(gvm_tsv_value(gvm::ctx(), p, imageIndex, &v), "gvm_tsv_value");
+  set_fivalue(v);
+  return (double)penalization_factor * v;
+}
+float TotalSquaredVariationP::calcFi(float* p) { return (float)calcFiD(p); }
+void TotalSquaredVariationP::calcGi(float* p, float* xi) {
+  if (prior_active(iteration, penalization_factor) && globals().flag_opt % 2 == imageIndex)
+    check(gvm_tsv_grad(gvm::ctx(), p, device_DS, imageIndex, penalization_factor), "gvm_tsv_grad");
+}
+void TotalSquaredVariationP::restartDGi() {
+  check(gvm_vec_zero(gvm::ctx(), device_DS, globals().M * globals().N), "gvm_vec_zero");
+}
+void TotalSquaredVariationP::addToDphi(float* device_dphi) {
+  check(gvm_add_to_dphi(gvm::ctx(), device_dphi, device_DS, imageToAdd), "AddToDPhi");
+}
+
+Laplacian::Laplacian() { this->name = "Laplacian"; }
+double Laplacian::calcFiD(float* p) {
+  double v = 0.0;
+  if (prior_active(iteration, penalization_factor))
+    check(gvm_laplacian_value(gvm::ctx(), p, imageIndex, &v), "gvm_laplacian_value");
+  set_fivalue(v);
+  return (double)penalization_factor * v;
+}
+float Laplacian::calcFi(float* p) { return (float)calcFiD(p); }
+void Laplacian::calcGi(float* p, float* xi) {
+  if (prior_active(iteration, penalization_factor) && globals().flag_opt % 2 == imageIndex)
+    check(gvm_laplacian_grad(gvm::ctx(), p, device_DS, imageIndex, penalization_factor),
+          "gvm_laplacian_grad");
+}
+void Laplacian::restartDGi() {
+  check(gvm_vec_zero(gvm::ctx(), device_DS, globals().M * globals().N), "gvm_vec_zero");
+}
+void Laplacian::addToDphi(float* device_dphi) {
+  check(gvm_add_to_dphi(gvm::ctx(), device_dphi, device_DS, imageToAdd), "AddToDPhi");
+}
+
+namespace {
+Fi* CreateChi2() { return new Chi2; }
+Fi* CreateEntropy() { return new Entropy; }
+Fi* CreateTSV() { return new TotalSquaredVariationP; }
+Fi* CreateLaplacian() { return new Laplacian; }
+const bool RegisteredChi2 = registerCreationFunction<Fi, std::string>("Chi2", CreateChi2);
+const bool RegisteredEntropy = registerCreationFunction<Fi, std::string>("Entropy", CreateEntropy);
+const bool RegisteredEntropyInt = registerCreationFunction<Fi, int>(0, CreateEntropy);
+const bool RegisteredTSV =
+    registerCreationFunction<Fi, std::string>("TotalSquaredVariation", CreateTSV);
+const bool RegisteredLaplacian =
+    registerCreationFunction<Fi, std::string>("Laplacian", CreateLaplacian);
+}  // namespace

@@ -1,0 +1,158 @@
+// C entry points of the framework (include/gvm/fw.h): composition of the
+// objective by factory id, evaluation, and optimisation, as src/main.cu
+// :147-203 and src/mfs.cu:978-1012 do it.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gvm/factory.h"
+#include "gvm/framework.h"
+#include "gvm/fw.h"
+#include "gvm/optimizers.h"
+#include "gvm/terms.h"
+
+struct gvm_fw_objective {
+  gvm_ctx* ctx = nullptr;
+  gvm_fw_config cfg{};
+  std::vector<std::unique_ptr<Fi>> terms;
+  std::unique_ptr<ObjectiveFunction> of;
+  imageMap* mapping = nullptr;
+  ~gvm_fw_objective() { delete[] mapping; }
+};
+
+namespace {
+thread_local std::string g_fw_error;
+
+gvm_status fail(const std::exception& e) {
+  g_fw_error = e.what();
+  if (auto* ge = dynamic_cast<const gvm::Error*>(&e)) return ge->status;
+  return GVM_ERR_ARG;
+}
+
+// Globals of the reference that the composition sets
+// (src/mfs.cu:158-180, src/functions.cu:181-301).
+void make_current(gvm_fw_objective* obj) {
+  auto& g = gvm::globals();
+  gvm_geometry geo;
+  gvm::check(gvm_get_geometry(obj->ctx, &geo), "gvm_get_geometry");
+  g.ctx = obj->ctx;
+  g.M = geo.M;
+  g.N = geo.N;
+  g.image_count = geo.image_count;
+  g.penalizators.assign(obj->cfg.lambdas, obj->cfg.lambdas + obj->cfg.nlambdas);
+  g.eta = obj->cfg.eta;
+  g.initial_values = {obj->cfg.initial_values[0], obj->cfg.initial_values[1]};
+  g.nopositivity = obj->cfg.nopositivity != 0;
+  g.flag_opt = obj->cfg.flag_opt;
+}
+}  // namespace
+
+extern "C" const char* gvm_fw_last_error(void) { return g_fw_error.c_str(); }
+
+extern "C" gvm_status gvm_fw_objective_create(gvm_ctx* ctx, const gvm_fw_config* cfg,
+                                              gvm_fw_objective** out) {
+  if (!ctx || !cfg || !out || cfg->nterms <= 0 || cfg->nterms > GVM_FW_MAX_TERMS ||
+      cfg->nlambdas < 0 || cfg->nlambdas > GVM_FW_MAX_TERMS) {
+    g_fw_error = "gvm_fw_objective_create: bad argument";
+    return GVM_ERR_ARG;
+  }
+  *out = nullptr;
+  auto obj = std::make_unique<gvm_fw_objective>();
+  obj->ctx = ctx;
+  obj->cfg = *cfg;
+  try {
+    make_current(obj.get());
+    obj->of.reset(createObject<ObjectiveFunction, std::string>("ObjectiveFunction"));
+    for (int t = 0; t < cfg->nterms; t++) {
+      const gvm_fw_term& tc = cfg->terms[t];
+      if (!tc.name) throw gvm::Error("term without a name", GVM_ERR_ARG);
+      std::unique_ptr<Fi> fi(createObject<Fi, std::string>(std::string(tc.name)));
+      fi->configure(tc.penalizator, tc.image_index, tc.image_to_add, tc.normalize != 0);
+      if (std::string(tc.name) == "Entropy") fi->setPrior(tc.prior);
+      obj->of->addFi(fi.get());
+      obj->terms.push_back(std::move(fi));
+    }
+    auto& g = gvm::globals();
+    obj->of->configure(g.N, g.M, g.image_count);
+    obj->mapping = makeFunctionMapping(g.image_count, g.nopositivity);
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  *out = obj.release();
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_objective_destroy(gvm_fw_objective* obj) {
+  delete obj;
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_value(gvm_fw_objective* obj, float* p_dev, double* value) {
+  if (!obj || !p_dev || !value) return GVM_ERR_ARG;
+  try {
+    make_current(obj);
+    *value = obj->of->calcFunctionD(p_dev);
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_gradient(gvm_fw_objective* obj, float* p_dev, float* xi_dev,
+                                      int iter) {
+  if (!obj || !p_dev || !xi_dev) return GVM_ERR_ARG;
+  try {
+    make_current(obj);
+    obj->of->calcGradient(p_dev, xi_dev, iter);
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_term_values(gvm_fw_objective* obj, double* values, int cap, int* n) {
+  if (!obj || !n) return GVM_ERR_ARG;
+  auto fis = obj->of->getFi();
+  *n = (int)fis.size();
+  for (int k = 0; k < (int)fis.size() && k < cap; k++) values[k] = fis[k]->get_fivalue_d();
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimizer,
+                                      int total_iterations, double ftol, double gtol, int lbfgs_k,
+                                      float* p_dev, gvm_fw_result* res, double* traj, int cap) {
+  if (!obj || !optimizer || !p_dev || !res || total_iterations < 0) return GVM_ERR_ARG;
+  try {
+    make_current(obj);
+    std::unique_ptr<Optimizer> opt(createObject<Optimizer, std::string>(std::string(optimizer)));
+    Image image(p_dev, gvm::globals().image_count);
+    image.setFunctionMapping(obj->mapping);
+    opt->setObjectiveFunction(obj->of.get());
+    opt->setTotalIterations(total_iterations);
+    opt->setFTol((float)ftol);
+    opt->setGTol((float)gtol);
+    if (lbfgs_k > 0) opt->setK(lbfgs_k);
+    opt->setImage(&image);
+    opt->setFlag(obj->cfg.flag_opt);
+    const long f0 = obj->of->getFunctionCalls(), g0 = obj->of->getGradientCalls();
+    const auto t0 = std::chrono::steady_clock::now();
+    opt->optimize();
+    gvm::check(gvm_sync(obj->ctx), "gvm_sync");
+    const auto t1 = std::chrono::steady_clock::now();
+    res->iterations = opt->getCurrentIteration();
+    res->exit_reason = opt->getExitReason();
+    res->final_value = opt->getFinalValue();
+    res->function_calls = obj->of->getFunctionCalls() - f0;
+    res->gradient_calls = obj->of->getGradientCalls() - g0;
+    res->seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (traj) {
+      const auto& tr = opt->getTrajectory();
+      for (int k = 0; k < (int)tr.size() && k < cap; k++) traj[k] = tr[k];
+    }
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return GVM_OK;
+}

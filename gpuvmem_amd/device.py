@@ -1,0 +1,228 @@
+"""Thin Python handle over a ``gvm_ctx`` (libgvm_hip C ABI) for harnesses.
+
+All device memory is allocated through the library (``gvm_malloc``); all
+computation runs in the HIP kernels of libgvm_hip.  Nothing here computes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import native
+from .problem import Problem
+
+
+class DeviceBuffer:
+    """A device allocation of ``nbytes`` made by gvm_malloc."""
+
+    def __init__(self, nbytes: int):
+        self.lib = native.load_hip()
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        native.check(self.lib.gvm_malloc(C.byref(p), self.nbytes), "gvm_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        buf.upload(arr)
+        return buf
+
+    def upload(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        native.check(self.lib.gvm_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes), "h2d")
+
+    def download(self, dtype=np.float32, shape=None) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        native.check(self.lib.gvm_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.lib.gvm_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A gvm_ctx with one slice per call to :meth:`add`."""
+
+    def __init__(self, problem: Problem, adjoint: str = "f16x3", add_data: bool = True):
+        self.lib = native.load_hip()
+        self.problem = problem
+        g = problem.geometry()
+        ctx = C.c_void_p()
+        noise = np.ascontiguousarray(problem.noise, np.float32)
+        native.check(self.lib.gvm_create(C.byref(g), noise.ctypes.data, C.byref(ctx)), "gvm_create")
+        self.ctx = ctx.value
+        self.set_adjoint(adjoint)
+        self.n = problem.M * problem.N
+        if add_data:
+            self.add(problem.freq, problem.uvw, problem.vo, problem.weight)
+
+    def add(self, freq: float, uvw: np.ndarray, vo: np.ndarray, weight: np.ndarray) -> int:
+        uvw = np.ascontiguousarray(uvw, np.float64)
+        vo = np.ascontiguousarray(vo, np.float32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        s = C.c_int()
+        native.check(self.lib.gvm_add_visibilities(self.ctx, freq, uvw.shape[0], uvw.ctypes.data,
+                                                   vo.ctypes.data, weight.ctypes.data, C.byref(s)),
+                     "gvm_add_visibilities")
+        return s.value
+
+    def set_adjoint(self, mode: str) -> None:
+        native.check(self.lib.gvm_set_adjoint(self.ctx, native.ADJOINT_MODES[mode]), "gvm_set_adjoint")
+
+    def image(self, img: np.ndarray) -> DeviceBuffer:
+        assert img.dtype == np.float32 and img.size == 2 * self.n
+        return DeviceBuffer.from_array(img)
+
+    def zeros(self, count: int) -> DeviceBuffer:
+        b = DeviceBuffer(4 * count)
+        native.check(self.lib.gvm_vec_zero(self.ctx, b.ptr, count), "gvm_vec_zero")
+        return b
+
+    def chi2(self, img: DeviceBuffer, flag_opt: int = 0, normalize: bool = False) -> float:
+        out = C.c_double()
+        native.check(self.lib.gvm_chi2(self.ctx, img.ptr, flag_opt, int(normalize), C.byref(out)),
+                     "gvm_chi2")
+        return out.value
+
+    def dchi2(self, img: DeviceBuffer, result: DeviceBuffer, flag_opt: int = 0,
+              normalize: bool = False) -> None:
+        native.check(self.lib.gvm_dchi2(self.ctx, img.ptr, result.ptr, flag_opt, int(normalize)),
+                     "gvm_dchi2")
+
+    def residuals(self, slice_index: int = 0, nvis: Optional[int] = None) -> np.ndarray:
+        nvis = self.problem.nvis if nvis is None else nvis
+        out = np.empty((nvis, 2), np.float32)
+        native.check(self.lib.gvm_get_residuals(self.ctx, slice_index, out.ctypes.data),
+                     "gvm_get_residuals")
+        return out
+
+    def prior_value(self, kind: str, img: DeviceBuffer, index: int = 0, prior: float = 0.001,
+                    eta: float = -1.0) -> float:
+        out = C.c_double()
+        if kind == "entropy":
+            st = self.lib.gvm_entropy_value(self.ctx, img.ptr, index, prior, eta, C.byref(out))
+        elif kind == "tsv":
+            st = self.lib.gvm_tsv_value(self.ctx, img.ptr, index, C.byref(out))
+        else:
+            st = self.lib.gvm_laplacian_value(self.ctx, img.ptr, index, C.byref(out))
+        native.check(st, f"{kind} value")
+        return out.value
+
+    def prior_grad(self, kind: str, img: DeviceBuffer, dgi: DeviceBuffer, index: int = 0,
+                   lam: float = 1.0, prior: float = 0.001, eta: float = -1.0) -> None:
+        if kind == "entropy":
+            st = self.lib.gvm_entropy_grad(self.ctx, img.ptr, dgi.ptr, index, prior, eta, lam)
+        elif kind == "tsv":
+            st = self.lib.gvm_tsv_grad(self.ctx, img.ptr, dgi.ptr, index, lam)
+        else:
+            st = self.lib.gvm_laplacian_grad(self.ctx, img.ptr, dgi.ptr, index, lam)
+        native.check(st, f"{kind} gradient")
+
+    def sync(self) -> None:
+        native.check(self.lib.gvm_sync(self.ctx), "gvm_sync")
+
+    def adjoint_time_ms(self) -> tuple[float, int]:
+        ms = C.c_double()
+        cnt = C.c_long()
+        native.check(self.lib.gvm_timing_adjoint_ms(self.ctx, C.byref(ms), C.byref(cnt)), "timing")
+        return ms.value, cnt.value
+
+    def reset_timing(self) -> None:
+        native.check(self.lib.gvm_timing_reset(self.ctx), "timing reset")
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.gvm_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# framework (libgvm): objective composed by factory id + optimisers
+# ---------------------------------------------------------------------------
+def fw_config(terms, lambdas=(), initial_values=(0.001, 0.0), eta=-1.0, flag_opt=0,
+              nopositivity=False) -> native.FwConfig:
+    """terms: list of (name, penalizator_index, prior) in the order of
+    src/main.cu:199-203 (Chi2 first)."""
+    cfg = native.FwConfig()
+    cfg.nterms = len(terms)
+    keep = []
+    for k, (name, pen, prior) in enumerate(terms):
+        b = name.encode()
+        keep.append(b)
+        cfg.terms[k] = native.FwTerm(b, pen, 0, 0, prior, 0)
+    cfg.nlambdas = len(lambdas)
+    for k, v in enumerate(lambdas):
+        cfg.lambdas[k] = v
+    cfg.initial_values[0], cfg.initial_values[1] = initial_values
+    cfg.eta = eta
+    cfg.nopositivity = int(nopositivity)
+    cfg.flag_opt = flag_opt
+    cfg._keep = keep  # keep the name buffers alive
+    return cfg
+
+
+class Objective:
+    """An ObjectiveFunction built through the C++ factories (libgvm)."""
+
+    def __init__(self, ctx: Context, cfg: native.FwConfig):
+        self.fw = native.load_fw()
+        self.ctx = ctx
+        self.cfg = cfg
+        h = C.c_void_p()
+        native.check_fw(self.fw.gvm_fw_objective_create(ctx.ctx, C.byref(cfg), C.byref(h)),
+                        "gvm_fw_objective_create")
+        self.h = h.value
+
+    def value(self, img: DeviceBuffer) -> float:
+        v = C.c_double()
+        native.check_fw(self.fw.gvm_fw_value(self.h, img.ptr, C.byref(v)), "gvm_fw_value")
+        return v.value
+
+    def gradient(self, img: DeviceBuffer, xi: DeviceBuffer, it: int) -> None:
+        native.check_fw(self.fw.gvm_fw_gradient(self.h, img.ptr, xi.ptr, it), "gvm_fw_gradient")
+
+    def term_values(self) -> list[float]:
+        vals = (C.c_double * 8)()
+        n = C.c_int()
+        native.check_fw(self.fw.gvm_fw_term_values(self.h, vals, 8, C.byref(n)), "term values")
+        return [vals[k] for k in range(n.value)]
+
+    def optimize(self, img: DeviceBuffer, optimizer: str = "CG-FRPRMN", iterations: int = 50,
+                 ftol: float = 1e-12, gtol: float = 1e-12, lbfgs_k: int = 0):
+        res = native.FwResult()
+        traj = (C.c_double * max(1, iterations))()
+        native.check_fw(self.fw.gvm_fw_optimize(self.h, optimizer.encode(), iterations, ftol, gtol,
+                                                lbfgs_k, img.ptr, C.byref(res), traj, iterations),
+                        "gvm_fw_optimize")
+        n = min(res.iterations, iterations)
+        return res, [traj[k] for k in range(n)]
+
+    def close(self) -> None:
+        if self.h:
+            self.fw.gvm_fw_objective_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,60 @@
+// Objective term interface: the reference's include/classes/fi.cuh:13-104
+// (same members and virtuals).  All float* crossing this interface are device
+// pointers to float[image_count*M*N].  calcFiD is an addition: the FP64 value
+// of the term (the default forwards to calcFi), which the optimisers use for
+// their line-search scalars.
+#pragma once
+
+#include <string>
+
+#include "gvm/framework.h"
+
+class CKernel;  // gridding kernels are not part of this path
+
+class Fi {
+ public:
+  Fi();
+  virtual ~Fi();
+
+  virtual float calcFi(float* p) = 0;
+  virtual void calcGi(float* p, float* xi) = 0;
+  virtual void restartDGi() = 0;
+  virtual void addToDphi(float* device_dphi) = 0;
+  virtual double calcFiD(float* p) { return (double)calcFi(p); }
+  virtual void setPrior(float prior) {}
+  virtual void setPrior(float* prior) {}
+  virtual float getEta() { return 0.0f; }
+  virtual void setEta(float eta) {}
+  virtual void setCKernel(CKernel* ckernel) {}
+  virtual void setFgScale(float fg_scale) {}
+  virtual float getFgScale() { return 0.0f; }
+
+  std::string getName() { return this->name; }
+  void setName(std::string name) { this->name = name; }
+
+  float get_fivalue() { return (float)this->fi_value; }
+  double get_fivalue_d() { return this->fi_value; }
+  bool getNormalize() { return this->normalize; }
+  float getPenalizationFactor() { return this->penalization_factor; }
+  void set_fivalue(double fi) { this->fi_value = fi; }
+  void setPenalizationFactor(float p) { this->penalization_factor = p; }
+  void setIteration(int iteration) { this->iteration = iteration; }
+  int getIteration() { return this->iteration; }
+  void setNormalize(bool normalize) { this->normalize = normalize; }
+
+  virtual float calculateSecondDerivate() { return 0.0f; }
+  // include/classes/fi.cuh:58-89: image indices, -Z lambda lookup, device_S /
+  // device_DS allocation.
+  virtual void configure(int penalizatorIndex, int imageIndex, int imageToAdd, bool normalize);
+
+ protected:
+  double fi_value = 0.0;
+  float* device_S = nullptr;
+  float* device_DS = nullptr;
+  float penalization_factor = 1.0f;
+  int imageIndex = 0;
+  int iteration = 0;
+  std::string name;
+  int imageToAdd = 0;
+  bool normalize = false;
+};

@@ -1,0 +1,74 @@
+/* C entry points of the C++ framework (libgvm): compose the objective from
+ * factory ids the way src/main.cu:147-203 does, evaluate it, and run an
+ * optimiser, all on a gvm_ctx.  For harnesses and bindings (Python ctypes,
+ * cgo, ...) that cannot instantiate the C++ classes themselves. */
+#ifndef GVM_FW_H
+#define GVM_FW_H
+
+#include "gvm_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GVM_FW_MAX_TERMS 8
+
+typedef struct {
+  const char* name;   /* factory id: "Chi2", "Entropy", "TotalSquaredVariation", "Laplacian" */
+  int penalizator;    /* index into lambdas, or -1 (keep the term's default lambda 1) */
+  int image_index;
+  int image_to_add;
+  float prior;        /* Entropy prior G (setPrior) */
+  int normalize;
+} gvm_fw_term;
+
+typedef struct {
+  int nterms;
+  gvm_fw_term terms[GVM_FW_MAX_TERMS];
+  int nlambdas;
+  float lambdas[GVM_FW_MAX_TERMS]; /* the -Z list */
+  float initial_values[2];         /* initial_values after src/mfs.cu:167-179 */
+  float eta;                       /* -e (default -1) */
+  int nopositivity;                /* -x */
+  int flag_opt;
+} gvm_fw_config;
+
+typedef struct gvm_fw_objective gvm_fw_objective;
+
+/* Makes ctx current, creates the terms via createObject<Fi,std::string>,
+ * configures them and adds them to an ObjectiveFunction ("ObjectiveFunction"). */
+gvm_status gvm_fw_objective_create(gvm_ctx* ctx, const gvm_fw_config* cfg,
+                                   gvm_fw_objective** out);
+gvm_status gvm_fw_objective_destroy(gvm_fw_objective* obj);
+/* ObjectiveFunction::calcFunction (FP64 sum of the terms). p_dev is clipped
+ * in place by the chi2 term, as in the reference. */
+gvm_status gvm_fw_value(gvm_fw_objective* obj, float* p_dev, double* value);
+/* ObjectiveFunction::calcGradient(p, xi, iter). */
+gvm_status gvm_fw_gradient(gvm_fw_objective* obj, float* p_dev, float* xi_dev, int iter);
+/* Values of the individual terms from the last gvm_fw_value (without lambda). */
+gvm_status gvm_fw_term_values(gvm_fw_objective* obj, double* values, int cap, int* n);
+
+typedef struct {
+  int iterations;
+  int exit_reason;
+  double final_value;
+  long function_calls;
+  long gradient_calls;
+  double seconds;
+} gvm_fw_result;
+
+/* Runs createObject<Optimizer,std::string>(optimizer) ("CG-FRPRMN" or
+ * "CG-LBFGS") on the image p_dev (float[2*M*N], device, updated in place),
+ * like optimizationOrder (src/main.cu:88-98).  lbfgs_k <= 0 keeps the
+ * default memory.  traj (optional, capacity cap) receives the value after
+ * each line search. */
+gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimizer, int total_iterations,
+                           double ftol, double gtol, int lbfgs_k, float* p_dev,
+                           gvm_fw_result* res, double* traj, int cap);
+
+const char* gvm_fw_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

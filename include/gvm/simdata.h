@@ -1,0 +1,62 @@
+/* Synthetic observation generator (host C, FP64, seeded).  It stands in for
+ * the Measurement Set reader of the reference (src/MSFITSIO.cu:398-754),
+ * which is not part of the accelerated path: it yields, for one field and one
+ * channel, uvw in metres, Vo, weights and the true sky. */
+#ifndef GVM_SIMDATA_H
+#define GVM_SIMDATA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GVM_SIM_LAYOUT_UNIFORM = 0, GVM_SIM_LAYOUT_GAUSSIAN = 1 };
+enum { GVM_SIM_FLUX_LOGUNIFORM = 0, GVM_SIM_FLUX_POWERLAW = 1 };
+
+typedef struct {
+  int npix;
+  double cell_arcsec;
+  int n_antennas;
+  int layout;
+  double array_diameter_m;
+  double layout_sigma_m;
+  double latitude_deg;
+  double declination_deg;
+  double ha_start_hours, ha_end_hours;
+  double dump_seconds;
+  double freq_hz;
+  double dish_diameter_m;
+  double pb_factor;
+  int n_gaussians;
+  int n_points;
+  double sigma_min_pix, sigma_max_pix;
+  int flux_law;
+  double flux_index;
+  double flux_min, flux_max;
+  double point_flux_min, point_flux_max;
+  double position_radius_frac;
+  double alpha_min, alpha_max;
+  double noise_sigma;
+  unsigned long long seed_layout, seed_sky, seed_noise;
+} gvm_sim_config;
+
+typedef struct {
+  double x_pix, y_pix;
+  double flux;
+  double sigma_pix; /* 0 for a point source */
+  double alpha;
+} gvm_sim_component;
+
+/* which: 0 = 256^2 / 36,000 vis workload, 1 = 1024^2 / 1,764,000 vis. */
+void gvm_sim_default_config(int which, gvm_sim_config* cfg);
+long gvm_sim_count_visibilities(const gvm_sim_config* cfg);
+/* uvw_m: 3*nvis doubles (metres); vo: 2*nvis floats; weight: nvis floats;
+ * sky (optional): npix*npix floats (Jy/pixel, unattenuated);
+ * comps_out (optional): n_gaussians + n_points entries.  Returns 0 on success. */
+int gvm_sim_generate(const gvm_sim_config* cfg, double* uvw_m, float* vo,
+                     float* weight, float* sky, gvm_sim_component* comps_out,
+                     int* ncomps_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

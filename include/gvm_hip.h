@@ -1,0 +1,245 @@
+/*
+ * gvm_hip.h -- C ABI of the MI355X (gfx950) implementation of gpuvmem's
+ * maximum-entropy objective and gradient.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes.  No
+ * HIP, torch or C++ types cross this boundary.  The C++ Fi / ObjectiveFunction
+ * / Optimizer classes (include/gvm/) are thin forwards to these calls.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the gpuvmem repository root).
+ *
+ * Conventions
+ *  - Images are device pointers to row-major float[image_count][N][M]
+ *    (index M*N*k + N*i + j), M == N, a power of two in [64, 4096].
+ *    image 0 is I_nu_0, image 1 is the spectral index alpha.
+ *  - uvw are in wavelengths after the Hermitian flip (u <= 0), as the
+ *    reference keeps them on the device after set-up.
+ *  - Calls on one context are stream-ordered on the context's own HIP stream
+ *    and are not re-entrant.  Calls that return a host scalar synchronise that
+ *    stream; the others only enqueue work.
+ *  - Device pointers passed in must stay valid until the call's work has
+ *    completed (gvm_sync).
+ */
+#ifndef GVM_HIP_H
+#define GVM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gvm_ctx gvm_ctx;
+typedef int gvm_status;
+
+/* Status codes.  The reference aborts on errors (checkCudaErrors, exit(-1));
+ * this ABI reports them instead. */
+#define GVM_OK 0
+#define GVM_ERR_ARG 1      /* invalid argument or unsupported size */
+#define GVM_ERR_OOM 2      /* device allocation failed */
+#define GVM_ERR_HIP 3      /* HIP runtime error (see gvm_last_error) */
+#define GVM_ERR_STATE 4    /* call out of order (e.g. dchi2 before chi2) */
+#define GVM_ERR_NODEV 5    /* no usable gfx950 device */
+
+/* Primary beam models: reference include/MSFITSIO.cuh:56 (AIRYDISK, GAUSSIAN). */
+#define GVM_BEAM_AIRYDISK 0
+#define GVM_BEAM_GAUSSIAN 1
+
+/* Adjoint (DChi2) evaluation forms. */
+#define GVM_ADJOINT_F32 0      /* separable complex GEMM, FP32 MFMA            */
+#define GVM_ADJOINT_F16X3 1    /* FP16 hi/lo split, 3 MFMA products, FP32 acc  */
+#define GVM_ADJOINT_BF16X3 2   /* BF16 hi/lo split, 3 MFMA products, FP32 acc  */
+#define GVM_ADJOINT_DIRECT 3   /* direct sum over visibilities per pixel (VALU) */
+
+/* Geometry and the scalars the reference keeps in globals
+ * (src/mfs.cu:4-38, src/functions.cu:37-71): image size, cell size,
+ * pointing/phase centres, nu_0, fg_scale, noise_cut, MINPIX
+ * (= initial_values[0]), eta, threshold, and the antenna beam
+ * (src/MSFITSIO.cu:522-552). */
+typedef struct {
+  int M, N;
+  int image_count;         /* must be 2 (I_nu_0, alpha), as src/mfs.cu:175-179 forces */
+  double deltax, deltay;   /* CDELT1/CDELT2 in degrees */
+  float x0_ref, y0_ref;    /* pointing centre in pixels */
+  float x0_phs, y0_phs;    /* phase centre in pixels */
+  float nu_0;              /* reference frequency (Hz) */
+  float fg_scale;
+  float noise_cut;         /* absolute (already scaled by the minimum noise) */
+  float minpix;            /* initial_values[0] */
+  float eta;               /* global eta (default -1) */
+  float threshold;         /* -T (default 0) */
+  int beam_kind;           /* GVM_BEAM_* */
+  float antenna_diameter;  /* metres */
+  float pb_factor;
+  float pb_cutoff;         /* radians */
+} gvm_geometry;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Replaces the device state built by MFS::setDevice (src/mfs.cu:530-943):
+ * copies the noise image (float[M*N], host) used as the mask, allocates the
+ * uv grid and work buffers, creates the context's stream. */
+gvm_status gvm_create(const gvm_geometry* geom, const float* noise_host,
+                      gvm_ctx** out);
+/* Replaces MFS::unSetDevice (src/mfs.cu:1157-1253). */
+gvm_status gvm_destroy(gvm_ctx* ctx);
+
+/* Replaces the per channel x stokes H2D copies of MFS::setDevice
+ * (src/mfs.cu:555-617) and the off-grid weight zeroing of vis_mod
+ * (src/functions.cu:2605-2606), done once here.  uvw: 3*nvis doubles
+ * (wavelengths, u <= 0), vo: 2*nvis floats, weight: nvis floats, all host.
+ * Returns the slice index.  Slices are evaluated in insertion order. */
+gvm_status gvm_add_visibilities(gvm_ctx* ctx, float freq, long nvis,
+                                const double* uvw, const float* vo,
+                                const float* weight, int* slice_out);
+gvm_status gvm_num_slices(const gvm_ctx* ctx, int* n);
+/* Copy of the geometry the context was created with. */
+gvm_status gvm_get_geometry(const gvm_ctx* ctx, gvm_geometry* out);
+
+/* Adjoint form used by gvm_dchi2 (GVM_ADJOINT_*; default GVM_ADJOINT_F16X3). */
+gvm_status gvm_set_adjoint(gvm_ctx* ctx, int mode);
+
+/* ---- objective: chi2 ----------------------------------------------------- */
+
+/* Replaces chi2() (src/functions.cu:4323-4454) as called by Chi2::calcFi
+ * (src/chi2.cu:44-49): clip2IWNoise in place on I (:2694-2719), then per
+ * slice calculateInu (:3939-3966), apply_beam2I (:2424-2444), the inverse 2-D
+ * FFT (:2153-2178), phase_rotate (:2483-2518), vis_mod (:2557-2609),
+ * residual (:2663-2671) and chi2Vector + reduction (:2867-2876).
+ * *chi2_out = 0.5 * sum_slices [sum_v w |Vo - Vm|^2 (/ nvis if normalize)],
+ * summed in FP64.  The residuals stay on the device for gvm_dchi2.
+ * flag_opt is the optimisation schedule flag (src/frprmn.cu:46). */
+gvm_status gvm_chi2(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize,
+                    double* chi2_out);
+
+/* Replaces dchi2() (src/functions.cu:4456-4558) as called by Chi2::calcGi
+ * (src/chi2.cu:51-53): per slice the DChi2 adjoint (:3698-3791) at the
+ * residuals of the last gvm_chi2, then DChi2_total_I_nu_0 (:4000-4024) when
+ * flag_opt is even or DChi2_total_alpha (:3968-3998) when odd, accumulated
+ * into result_dev (float[image_count*M*N], device).  result_dev is not
+ * cleared first (Chi2::restartDGi does that). */
+gvm_status gvm_dchi2(gvm_ctx* ctx, const float* I_dev, float* result_dev,
+                     int flag_opt, int normalize);
+
+/* Residuals V_r of one slice from the last gvm_chi2 (host float[2*nvis]);
+ * replaces the Vr readback of MFS::writeResiduals (src/mfs.cu:1115-1155). */
+gvm_status gvm_get_residuals(gvm_ctx* ctx, int slice, float* vr_host);
+
+/* ---- priors ------------------------------------------------------------- */
+/* Values: sum over the mask of the per-pixel term on image `index`, FP64.
+ * Gradients: overwrite dgi_dev (float[M*N]) with lambda * d(term).
+ * The iter > 0 and flag_opt gating of the reference host launchers is the
+ * caller's (the Fi classes') job. */
+
+/* SEntropy -> SVector/calculateS (src/functions.cu:4722-4742, 3081-3095,
+ * 3036-3055). */
+gvm_status gvm_entropy_value(gvm_ctx* ctx, const float* I_dev, int index,
+                             float prior, float eta, double* out);
+/* DEntropy -> DS/calculateDS (src/functions.cu:4744-4763, 3097-3112,
+ * 3057-3079). */
+gvm_status gvm_entropy_grad(gvm_ctx* ctx, const float* I_dev, float* dgi_dev,
+                            int index, float prior, float eta, float lambda);
+/* TotalSquaredVariation -> TSVVector/calculateTSV (src/functions.cu:4927-4945,
+ * 3382-3393, 3353-3380). */
+gvm_status gvm_tsv_value(gvm_ctx* ctx, const float* I_dev, int index,
+                         double* out);
+/* DTSVariation -> DTSV/calculateDTSV (src/functions.cu:4947-4964, 3427-3441,
+ * 3395-3425). */
+gvm_status gvm_tsv_grad(gvm_ctx* ctx, const float* I_dev, float* dgi_dev,
+                        int index, float lambda);
+/* laplacian -> LVector/calculateL (src/functions.cu:4808-4826, 3475-3486,
+ * 3443-3473). */
+gvm_status gvm_laplacian_value(gvm_ctx* ctx, const float* I_dev, int index,
+                               double* out);
+/* DLaplacian -> DL/calculateDL (src/functions.cu:4828-4845, 3532-3545,
+ * 3488-3530), including the reference's stencil sign pattern. */
+gvm_status gvm_laplacian_grad(gvm_ctx* ctx, const float* I_dev,
+                              float* dgi_dev, int index, float lambda);
+
+/* linkAddToDPhi -> AddToDPhi (src/functions.cu:4560-4564, 3890-3895):
+ * dphi[index] += dgi. */
+gvm_status gvm_add_to_dphi(gvm_ctx* ctx, float* dphi_dev, const float* dgi_dev,
+                           int index);
+
+/* ---- optimiser vector operations (all on float[nimg*M*N] device vectors;
+ *      reductions are FP64) ------------------------------------------------ */
+
+/* cudaMemset of the reference, as a fill kernel (graph-capture safe). */
+gvm_status gvm_vec_zero(gvm_ctx* ctx, float* x_dev, long n);
+/* cudaMemcpy device-to-device of the reference. */
+gvm_status gvm_vec_copy(gvm_ctx* ctx, float* dst_dev, const float* src_dev,
+                        long n);
+/* searchDirection (src/functions.cu:3655-3668): g = -xi; xi = h = g, image k. */
+gvm_status gvm_cg_search_direction(gvm_ctx* ctx, float* g, float* xi, float* h,
+                                   int image);
+/* CGGradCondition + deviceMaxReduce (src/functions.cu:3599-3612, 743-879):
+ * max over all nimg images of |xi| * max(|p|, 1) / den. */
+gvm_status gvm_cg_grad_condition(gvm_ctx* ctx, const float* xi, const float* p,
+                                 double den, int nimg, double* gmax);
+/* getGGandDGG + 2 x deviceReduce (src/functions.cu:2729-2749):
+ * gg = sum g^2, dgg = sum (xi + g) xi over all nimg images. */
+gvm_status gvm_cg_gg_dgg(gvm_ctx* ctx, const float* xi, const float* g,
+                         int nimg, double* gg, double* dgg);
+/* newXi (src/functions.cu:3678-3687): g = -xi; xi = h = g + gam h, image k. */
+gvm_status gvm_cg_new_xi(gvm_ctx* ctx, float* g, float* xi, float* h,
+                         double gam, int image);
+/* evaluateXt / evaluateXtNoPositivity (src/functions.cu:2832-2865):
+ * xt = pcom + x xicom on image k, floored at -eta*minpix if positivity. */
+gvm_status gvm_evaluate_xt(gvm_ctx* ctx, float* xt, const float* pcom,
+                           const float* xicom, double x, int image,
+                           int positivity, float minpix, float eta);
+/* newP / newPNoPositivity (src/functions.cu:2779-2812): xi *= xmin;
+ * p += xi, or with positivity p = floor and xi = 0 where p + xi <= floor. */
+gvm_status gvm_new_p(gvm_ctx* ctx, float* p, float* xi, double xmin, int image,
+                     int positivity, float minpix, float eta);
+/* normArray + deviceMaxReduce (src/functions.cu:3587-3597): max |x| over n. */
+gvm_status gvm_vec_absmax(gvm_ctx* ctx, const float* x, long n, double* out);
+/* searchDirection_LBFGS / getR (src/functions.cu:3564-3569, 3623-3633):
+ * dst = src * scale over n elements (dst may alias src). */
+gvm_status gvm_vec_scale(gvm_ctx* ctx, float* dst, const float* src,
+                         double scale, long n);
+/* getDot_LBFGS_ff + deviceReduce (src/functions.cu:3571-3585): sum a*b, FP64. */
+gvm_status gvm_vec_dot(gvm_ctx* ctx, const float* a, const float* b, long n,
+                       double* out);
+/* updateQ (src/functions.cu:3614-3621): y += alpha x over n elements. */
+gvm_status gvm_vec_axpy(gvm_ctx* ctx, float* y, const float* x, double alpha,
+                        long n);
+/* calculateSandY (src/functions.cu:3635-3653) for one image:
+ * y = xi + xi_old; s = p - p_old over n elements. */
+gvm_status gvm_lbfgs_s_and_y(gvm_ctx* ctx, float* y, float* s, const float* p,
+                             const float* xi, const float* p_old,
+                             const float* xi_old, long n);
+
+/* ---- set-up helpers (host, FP64) ---------------------------------------- */
+
+/* attenuation (src/functions.cu:2304-2333) with GaussianBeam / AiryDiskBeam
+ * (:2293-2302, 2275-2291) for every pixel, host float[M*N]. */
+gvm_status gvm_primary_beam(int beam_kind, double antenna_diameter,
+                            double pb_factor, double pb_cutoff, double freq,
+                            int M, int N, float xobs, float yobs,
+                            double deltax, double deltay, float* out);
+
+/* ---- runtime ------------------------------------------------------------ */
+gvm_status gvm_sync(gvm_ctx* ctx);
+/* Device memory helpers for callers without their own allocator. */
+gvm_status gvm_malloc(void** ptr, size_t bytes);
+gvm_status gvm_free(void* ptr);
+gvm_status gvm_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+gvm_status gvm_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+/* Text of the last error on this thread. */
+const char* gvm_last_error(void);
+/* Device name, CU count and clock of the current device; 0 if none. */
+gvm_status gvm_device_info(char* name, int name_len, int* cu_count,
+                           int* clock_khz);
+
+/* ---- timing of the adjoint (for measurement) ---------------------------- */
+/* Average duration in ms of the adjoint GEMM kernel over the gvm_dchi2 calls
+ * since the last reset, measured with HIP events on the context's stream. */
+gvm_status gvm_timing_reset(gvm_ctx* ctx);
+gvm_status gvm_timing_adjoint_ms(gvm_ctx* ctx, double* avg_ms, long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GVM_HIP_H */

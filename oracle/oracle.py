@@ -1,0 +1,314 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+the cpu_baseline leg of bench.py, always as the checker (or as the timed CPU
+baseline), never by the product path.  Every function restates the reference
+(file:line citations are in gvm_oracle.c).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+class OCplx(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class OGeom(C.Structure):
+    _fields_ = [
+        ("M", C.c_int), ("N", C.c_int),
+        ("deltax", C.c_double), ("deltay", C.c_double),
+        ("deltau", C.c_double), ("deltav", C.c_double),
+        ("x0_ref", C.c_float), ("y0_ref", C.c_float),
+        ("x0_phs", C.c_float), ("y0_phs", C.c_float),
+        ("fg_scale", C.c_float), ("noise_cut", C.c_float), ("minpix", C.c_float),
+        ("eta", C.c_float), ("threshold", C.c_float), ("nu_0", C.c_float),
+        ("noise", C.c_void_p),
+    ]
+
+
+class OBeam(C.Structure):
+    _fields_ = [("kind", C.c_int), ("diameter", C.c_double), ("pb_factor", C.c_double),
+                ("pb_cutoff", C.c_double)]
+
+
+class OChan(C.Structure):
+    _fields_ = [("freq", C.c_float), ("nvis", C.c_long), ("uvw", C.c_void_p),
+                ("vo", C.c_void_p), ("weight", C.c_void_p)]
+
+
+class OFi(C.Structure):
+    _fields_ = [("kind", C.c_int), ("lam", C.c_double), ("imageIndex", C.c_int),
+                ("imageToAdd", C.c_int), ("prior", C.c_double), ("eta", C.c_double),
+                ("iteration", C.c_int), ("value", C.c_double)]
+
+
+class OObjective(C.Structure):
+    _fields_ = [
+        ("g", C.POINTER(OGeom)), ("b", C.POINTER(OBeam)), ("ch", C.POINTER(OChan)),
+        ("nch", C.c_int), ("image_count", C.c_int), ("flag_opt", C.c_int),
+        ("normalize", C.c_int), ("separable", C.c_int), ("nopositivity", C.c_int),
+        ("nfi", C.c_int), ("fi", OFi * 8), ("vr", C.c_void_p), ("dphi", C.c_void_p),
+        ("dgi", C.c_void_p), ("n_fcalls", C.c_long), ("n_gcalls", C.c_long),
+    ]
+
+
+class OResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("exit_reason", C.c_int), ("fret", C.c_double),
+                ("final_value", C.c_double), ("n_fcalls", C.c_long), ("n_gcalls", C.c_long)]
+
+
+F1 = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+FI_KIND = {"Chi2": 0, "Entropy": 1, "TotalSquaredVariation": 2, "Laplacian": 3}
+
+_lib = None
+
+
+def build() -> None:
+    """Compiles liboracle.so with oracle/Makefile."""
+    subprocess.run(["make", "-C", str(HERE), "-s"], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P, D, I = C.c_void_p, C.c_double, C.c_int
+        GP, BP, CP = C.POINTER(OGeom), C.POINTER(OBeam), C.POINTER(OChan)
+        sig = {
+            "orc_fft2d": ([P, I, I, I], I),
+            "orc_clip2IWNoise": ([GP, P, I], None),
+            "orc_model_grid": ([GP, BP, P, C.c_float, P], None),
+            "orc_degrid_chi2": ([GP, P, CP, P, P], D),
+            "orc_chi2": ([GP, BP, CP, I, P, I, I, P], D),
+            "orc_dchi2_channel": ([GP, BP, CP, P, I, I, P], None),
+            "orc_dchi2_direct_pixels": ([GP, BP, CP, P, I, P, I, P], None),
+            "orc_dchi2": ([GP, BP, CP, I, P, P, P, I, I, I], None),
+            "orc_entropy_value": ([GP, P, I, D, D], D),
+            "orc_entropy_grad": ([GP, P, I, D, D, D, P], None),
+            "orc_tsv_value": ([GP, P, I], D),
+            "orc_tsv_grad": ([GP, P, I, D, P], None),
+            "orc_laplacian_value": ([GP, P, I], D),
+            "orc_laplacian_grad": ([GP, P, I, D, P], None),
+            "orc_attenuation": ([BP, D, I, I, C.c_float, C.c_float, D, D], D),
+            "orc_pb_image": ([BP, D, I, I, C.c_float, C.c_float, D, D, P], None),
+            "orc_hermitian_to_lambda": ([C.c_long, P, P, C.c_float], None),
+            "orc_beam_and_noise": ([CP, I, D, P, P, P, P, P], None),
+            "orc_noise_jypix": ([D, D, D, D], D),
+            "orc_noise_image": ([BP, C.c_float, I, I, D, D, C.c_float, C.c_float, D, P, P], None),
+            "orc_of_init": ([C.POINTER(OObjective)], I),
+            "orc_of_free": ([C.POINTER(OObjective)], None),
+            "orc_of_calc_function": ([C.POINTER(OObjective), P], D),
+            "orc_of_calc_gradient": ([C.POINTER(OObjective), P, P, I], None),
+            "orc_mnbrak": ([P, P, P, P, P, P, F1, P], None),
+            "orc_brent": ([D, D, D, D, P, F1, P], D),
+            "orc_linmin": ([C.POINTER(OObjective), P, P], D),
+            "orc_frprmn": ([C.POINTER(OObjective), P, I, D, D, C.POINTER(OResult)], I),
+            "orc_lbfgs": ([C.POINTER(OObjective), P, I, I, D, D, C.POINTER(OResult)], I),
+            "orc_do_gridding_pillbox": ([I, I, D, D, CP, I, I, I, P, P, P], None),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def fft2d(a: np.ndarray, sign: int = +1) -> np.ndarray:
+    """Unnormalised 2-D DFT with e^{sign 2 pi i} (power-of-two sizes)."""
+    x = np.ascontiguousarray(a, np.complex128).copy()
+    rc = lib().orc_fft2d(_ptr(x), x.shape[0], x.shape[1], sign)
+    if rc != 0:
+        raise ValueError("oracle FFT needs power-of-two sizes")
+    return x
+
+
+def mnbrak(f, ax: float, bx: float):
+    cb = F1(lambda x, _: float(f(x)))
+    v = [C.c_double(ax), C.c_double(bx), C.c_double(), C.c_double(), C.c_double(), C.c_double()]
+    lib().orc_mnbrak(*[C.addressof(t) for t in v], cb, None)
+    return tuple(t.value for t in v)
+
+
+def brent(f, ax: float, bx: float, cx: float, tol: float = 1e-7):
+    cb = F1(lambda x, _: float(f(x)))
+    xmin = C.c_double()
+    fx = lib().orc_brent(ax, bx, cx, tol, C.addressof(xmin), cb, None)
+    return xmin.value, fx
+
+
+class Oracle:
+    """The hot path of one problem (single channel) in FP64.
+
+    ``noise`` defaults to the problem's float32 noise image so the mask is
+    bit-identical to the one the device uses."""
+
+    def __init__(self, problem, separable: bool = True, noise=None):
+        self.p = problem
+        L = lib()
+        self.noise = np.ascontiguousarray(problem.noise if noise is None else noise, np.float32)
+        dxr = problem.deltax * np.pi / 180.0
+        dyr = problem.deltay * np.pi / 180.0
+        self.geom = OGeom(problem.M, problem.N, problem.deltax, problem.deltay,
+                          1.0 / (problem.M * dxr), 1.0 / (problem.N * dyr),
+                          problem.x0, problem.y0, problem.x0, problem.y0,
+                          problem.fg_scale, problem.noise_cut, problem.minpix, problem.eta,
+                          problem.threshold, problem.nu_0, _ptr(self.noise))
+        self.beam = OBeam(problem.beam_kind, problem.antenna_diameter, problem.pb_factor,
+                          problem.pb_cutoff)
+        self.uvw = np.ascontiguousarray(problem.uvw, np.float64)
+        self.vo = np.ascontiguousarray(problem.vo, np.float32)
+        self.weight = np.ascontiguousarray(problem.weight, np.float32).copy()
+        self.chan = (OChan * 1)(OChan(problem.freq, self.uvw.shape[0], _ptr(self.uvw),
+                                      _ptr(self.vo), _ptr(self.weight)))
+        self.separable = separable
+        self.n = problem.M * problem.N
+        self._lib = L
+
+    # ---- forward ----
+    def chi2(self, image, flag_opt: int = 0, normalize: bool = False):
+        """Returns (chi2, residuals complex128, clipped image float64)."""
+        img = np.ascontiguousarray(image, np.float64).reshape(2, self.p.N, self.p.M).copy()
+        vr = np.zeros(self.uvw.shape[0], np.complex128)
+        v = self._lib.orc_chi2(C.byref(self.geom), C.byref(self.beam), self.chan, 1, _ptr(img),
+                               flag_opt, int(normalize), _ptr(vr))
+        return v, vr, img
+
+    def model_grid(self, image):
+        img = np.ascontiguousarray(image, np.float64).reshape(2, self.p.N, self.p.M).copy()
+        grid = np.zeros((self.p.N, self.p.M), np.complex128)
+        self._lib.orc_model_grid(C.byref(self.geom), C.byref(self.beam), _ptr(img), self.p.freq,
+                                 _ptr(grid))
+        return grid
+
+    def pb(self, freq=None):
+        out = np.zeros((self.p.N, self.p.M), np.float64)
+        self._lib.orc_pb_image(C.byref(self.beam), self.p.freq if freq is None else freq,
+                               self.p.M, self.p.N, self.p.x0, self.p.y0, self.p.deltax,
+                               self.p.deltay, _ptr(out))
+        return out
+
+    # ---- adjoint ----
+    def dchi2(self, image, vr, flag_opt: int = 0, normalize: bool = False, separable=None):
+        img = np.ascontiguousarray(image, np.float64).reshape(2, self.p.N, self.p.M)
+        vr = np.ascontiguousarray(vr, np.complex128)
+        res = np.zeros((2, self.p.N, self.p.M), np.float64)
+        sep = self.separable if separable is None else separable
+        self._lib.orc_dchi2(C.byref(self.geom), C.byref(self.beam), self.chan, 1, _ptr(img),
+                            _ptr(vr), _ptr(res), flag_opt, int(normalize), int(sep))
+        return res
+
+    def dchi2_channel(self, vr, normalize: bool = False, separable=None):
+        vr = np.ascontiguousarray(vr, np.complex128)
+        out = np.zeros((self.p.N, self.p.M), np.float64)
+        sep = self.separable if separable is None else separable
+        self._lib.orc_dchi2_channel(C.byref(self.geom), C.byref(self.beam), self.chan, _ptr(vr),
+                                    int(normalize), int(sep), _ptr(out))
+        return out
+
+    def dchi2_pixels(self, vr, pixels, normalize: bool = False):
+        vr = np.ascontiguousarray(vr, np.complex128)
+        pix = np.ascontiguousarray(np.asarray(pixels, np.int32).reshape(-1, 2))
+        out = np.zeros(pix.shape[0], np.float64)
+        self._lib.orc_dchi2_direct_pixels(C.byref(self.geom), C.byref(self.beam), self.chan,
+                                          _ptr(vr), int(normalize), _ptr(pix), pix.shape[0],
+                                          _ptr(out))
+        return out
+
+    # ---- priors ----
+    def prior_value(self, kind: str, image, index: int = 0, prior: float = 0.001,
+                    eta: float = -1.0) -> float:
+        img = np.ascontiguousarray(image, np.float64)
+        g = C.byref(self.geom)
+        if kind == "entropy":
+            return self._lib.orc_entropy_value(g, _ptr(img), index, prior, eta)
+        if kind == "tsv":
+            return self._lib.orc_tsv_value(g, _ptr(img), index)
+        return self._lib.orc_laplacian_value(g, _ptr(img), index)
+
+    def prior_grad(self, kind: str, image, index: int = 0, lam: float = 1.0,
+                   prior: float = 0.001, eta: float = -1.0):
+        img = np.ascontiguousarray(image, np.float64)
+        out = np.zeros((self.p.N, self.p.M), np.float64)
+        g = C.byref(self.geom)
+        if kind == "entropy":
+            self._lib.orc_entropy_grad(g, _ptr(img), index, prior, eta, lam, _ptr(out))
+        elif kind == "tsv":
+            self._lib.orc_tsv_grad(g, _ptr(img), index, lam, _ptr(out))
+        else:
+            self._lib.orc_laplacian_grad(g, _ptr(img), index, lam, _ptr(out))
+        return out
+
+    # ---- objective + optimisers ----
+    def objective(self, terms, lambdas=(), flag_opt: int = 0, nopositivity: bool = False):
+        """terms: list of (name, penalizator_index, prior), like
+        gpuvmem_amd.device.fw_config."""
+        of = OObjective()
+        of.g = C.pointer(self.geom)
+        of.b = C.pointer(self.beam)
+        of.ch = C.cast(self.chan, C.POINTER(OChan))
+        of.nch = 1
+        of.image_count = 2
+        of.flag_opt = flag_opt
+        of.normalize = 0
+        of.separable = int(self.separable)
+        of.nopositivity = int(nopositivity)
+        k = 0
+        for name, pen, prior in terms:
+            if name == "Chi2":
+                lam = 1.0 if pen == -1 else lambdas[pen]
+            else:
+                lam = lambdas[pen] if 0 <= pen < len(lambdas) else 0.0
+            if lam == 0.0:
+                continue  # ObjectiveFunction::addFi drops zero-lambda terms
+            of.fi[k] = OFi(FI_KIND[name], lam, 0, 0, prior, -1.0, 0, 0.0)
+            k += 1
+        of.nfi = k
+        rc = self._lib.orc_of_init(C.byref(of))
+        if rc != 0:
+            raise MemoryError("oracle objective allocation failed")
+        return of
+
+    def free(self, of) -> None:
+        self._lib.orc_of_free(C.byref(of))
+
+    def value(self, of, image) -> tuple[float, np.ndarray]:
+        img = np.ascontiguousarray(image, np.float64).copy()
+        v = self._lib.orc_of_calc_function(C.byref(of), _ptr(img))
+        return v, img
+
+    def gradient(self, of, image, it: int) -> np.ndarray:
+        img = np.ascontiguousarray(image, np.float64).copy()
+        xi = np.zeros_like(img)
+        self._lib.orc_of_calc_gradient(C.byref(of), _ptr(img), _ptr(xi), it)
+        return xi
+
+    def optimize(self, of, image, optimizer: str = "CG-FRPRMN", iterations: int = 50,
+                 ftol: float = 1e-12, gtol: float = 1e-12, lbfgs_k: int = 100):
+        img = np.ascontiguousarray(image, np.float64).copy()
+        res = OResult()
+        if optimizer == "CG-FRPRMN":
+            self._lib.orc_frprmn(C.byref(of), _ptr(img), iterations, ftol, gtol, C.byref(res))
+        else:
+            self._lib.orc_lbfgs(C.byref(of), _ptr(img), lbfgs_k, iterations, ftol, gtol,
+                                C.byref(res))
+        return img, res
+
+
+def set_threads(n: int) -> None:
+    os.environ["OMP_NUM_THREADS"] = str(n)
