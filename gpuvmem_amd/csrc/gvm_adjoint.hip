@@ -68,25 +68,6 @@ __device__ __forceinline__ float bf16_to_f(unsigned short h) {
   return __uint_as_float(((unsigned int)h) << 16);
 }
 
-template <int PREC>
-struct AdjLds;
-
-template <>
-struct AdjLds<PREC_F32> {
-  static constexpr int kStride = kKP + 4;  // floats per row (144 B)
-  float a[kTile * kStride];
-  float b[kTile * kStride];
-};
-
-template <int PREC>
-struct AdjLds {
-  static constexpr int kStride = kKP + 8;  // 16-bit elements per row (80 B)
-  unsigned short ah[kTile * kStride];
-  unsigned short al[kTile * kStride];
-  unsigned short bh[kTile * kStride];
-  unsigned short bl[kTile * kStride];
-};
-
 // Splits x into hi + lo in the 16-bit format of PREC.
 template <int PREC>
 __device__ __forceinline__ void split16(float x, unsigned short* hi, unsigned short* lo) {
@@ -111,18 +92,60 @@ __device__ __forceinline__ f32x16_t mfma16(const u16x8_t& a, const u16x8_t& b, f
                                                  __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
 }
 
-// One workgroup: 256 threads (4 waves) own a 256x256 tile of G for one
-// K-range.  Wave w computes rows (w/2)*128.. and columns (w%2)*128.. as 4x4
-// blocks of 32x32 MFMA outputs (256 FP32 accumulators per lane).
+// LDS of the GEMM kernel.  Two buffers of P (rows) and Q (columns) tiles,
+// 256 rows x kKP values each; 16-bit precisions keep hi and lo parts in
+// separate arrays.  Rows are addressed in 16-byte groups with an XOR swizzle
+// so that the 32 rows read by one MFMA operand load spread over the banks.
+// Plus two buffers of visibility records for the chunk being generated.
 template <int PREC>
-__global__ __launch_bounds__(256, 1) void adjoint_gemm_kernel(
+struct AdjLds {
+  static constexpr int kRowBytes = 64;           // 32 x 16-bit
+  static constexpr int kGroups = kRowBytes / 16;  // 4
+  unsigned char ah[2][kTile * kRowBytes];
+  unsigned char al[2][kTile * kRowBytes];
+  unsigned char bh[2][kTile * kRowBytes];
+  unsigned char bl[2][kTile * kRowBytes];
+  float4 rec_s[2][kKV];  // (a_hi, a_lo, b_hi, b_lo)
+  float4 rec_d[2][kKV];  // (scale w Re Vr, -scale w Im Vr, w, 0)
+};
+
+template <>
+struct AdjLds<PREC_F32> {
+  static constexpr int kRowBytes = 128;           // 32 x float
+  static constexpr int kGroups = kRowBytes / 16;  // 8
+  unsigned char a[2][kTile * kRowBytes];
+  unsigned char b[2][kTile * kRowBytes];
+  float4 rec_s[2][kKV];
+  float4 rec_d[2][kKV];
+};
+
+template <int NG>
+__device__ __forceinline__ int swz(int row, int g) {
+  return row * (NG * 16) + ((g ^ ((row >> 2) & (NG - 1))) << 4);
+}
+
+constexpr int kGemmThreads = 512;
+
+// One workgroup (8 waves, 2 per SIMD) owns a 256x256 tile of G for one
+// K-range of visibilities.  Wave w accumulates rows (w&1)*128.. and columns
+// (w>>1)*64.. as 4x2 blocks of 32x32 MFMA outputs (128 FP32 accumulators per
+// lane).  Per chunk of kKV visibilities:
+//   threads   0..255 (waves 0-3) generate P row t   (c Ey, with the chi2 weight),
+//   threads 256..511 (waves 4-7) generate Q column t (Ex),
+// into the other LDS buffer while the chunk before is multiplied.  Waves 0-3
+// generate first and multiply second, waves 4-7 the other way round, so on
+// each SIMD one wave issues VALU work while the other keeps the MFMA pipe
+// busy; one barrier per chunk.  Visibility records are loaded from HBM two
+// chunks ahead by 32 lanes of wave 0 and staged in LDS.
+template <int PREC>
+__global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
     const float4* __restrict__ phs, const float* __restrict__ wcoord,
     const float* __restrict__ weight, const float2* __restrict__ vr, long nvis,
     const float2* __restrict__ xtab, const float2* __restrict__ ytab, int N, int tiles_x,
     int ntiles, int nsplit, long kper, const unsigned int* __restrict__ cmax,
     float* __restrict__ part) {
   __shared__ AdjLds<PREC> sm;
-  constexpr int S = AdjLds<PREC>::kStride;
+  constexpr int NG = AdjLds<PREC>::kGroups;
 
   // XCD-aware order: the workgroups of one K-range (which read the same
   // visibilities) are dispatched to the same XCD (blockIdx % 8) and share its L2.
@@ -135,147 +158,175 @@ __global__ __launch_bounds__(256, 1) void adjoint_gemm_kernel(
 
   const long kbeg = (long)split * kper;
   const long kend = min(nvis, kbeg + kper);
+  const int nchunks = kend > kbeg ? (int)((kend - kbeg + kKV - 1) / kKV) : 0;
 
   const int t = threadIdx.x;
-  const int row = ti * kTile + t, col = tj * kTile + t;
-  const bool rvalid = row < N, cvalid = col < N;
-  const float2 yt = rvalid ? ytab[row] : make_float2(0.f, 0.f);
-  const float2 xt = cvalid ? xtab[col] : make_float2(0.f, 0.f);
+  const int lane = t & 63, wave = t >> 6;
+  const bool group_b = wave >= 4;  // Q generators; multiply first
+  const int gidx = t & (kTile - 1);
+  // generator constants: row (P) or column (Q) of this thread
+  bool gvalid;
+  float2 gtab;
+  if (!group_b) {
+    const int row = ti * kTile + gidx;
+    gvalid = row < N;
+    gtab = gvalid ? ytab[row] : make_float2(0.f, 0.f);
+  } else {
+    const int col = tj * kTile + gidx;
+    gvalid = col < N;
+    gtab = gvalid ? xtab[col] : make_float2(0.f, 0.f);
+  }
 
   const float cm = __uint_as_float(*cmax);
   const float scale = cm > 0.0f ? exp2f(-ceilf(log2f(cm))) : 1.0f;
 
-  const int lane = t & 63, wave = t >> 6;
   const int lr = lane & 31, lh = lane >> 5;
-  const int wr = (wave >> 1) * 128, wc = (wave & 1) * 128;
+  const int wr = (wave & 1) * 128, wc = (wave >> 1) * 64;
 
-  f32x16_t acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-      for (int e = 0; e < 16; e++) acc[a][b][e] = 0.0f;
+  // ---- visibility records (wave 0, lanes 0..31) ----
+  const bool loader = t < 2 * kKV;
+  auto load_rec = [&](int c) -> float4 {
+    const long k = kbeg + (long)c * kKV + (lane & (kKV - 1));
+    if (k >= kend) return make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < kKV) return phs[k];
+    const float2 r = vr[k];
+    const float w = weight[k] * scale;
+    return make_float4(w * r.x, -w * r.y, wcoord[k], 0.f);
+  };
+  auto store_rec = [&](int buf, float4 v) {
+    if (lane < kKV)
+      sm.rec_s[buf][lane] = v;
+    else
+      sm.rec_d[buf][lane - kKV] = v;
+  };
 
-  for (long k0 = kbeg; k0 < kend; k0 += kKV) {
-    // ---- generate P (row t) and Q (column t) for kKV visibilities ----
-    float pv[kKP], qv[kKP];
+  // ---- P / Q generation for chunk c into buffer buf ----
+  auto generate = [&](int buf) {
+    float v[kKP];
 #pragma unroll
     for (int kk = 0; kk < kKV; kk++) {
-      const long k = k0 + kk;
-      float4 ph = make_float4(0.f, 0.f, 0.f, 0.f);
-      float wcv = 0.f, cr = 0.f, ci = 0.f;
-      if (k < kend) {
-        ph = phs[k];
-        wcv = wcoord[k];
-        const float wt = weight[k] * scale;
-        const float2 r = vr[k];
-        cr = wt * r.x;
-        ci = -wt * r.y;
+      const float4 s = sm.rec_s[buf][kk];
+      const float4 d = sm.rec_d[buf][kk];
+      float sn, cs;
+      if (!group_b) {
+        const float ph = fract(s.z * gtab.x) + fmaf(s.w, gtab.x, d.z * gtab.y);
+        sincos_turns(ph, &sn, &cs);
+        v[2 * kk] = gvalid ? d.x * cs - d.y * sn : 0.0f;
+        v[2 * kk + 1] = gvalid ? -(d.x * sn + d.y * cs) : 0.0f;
+      } else {
+        const float ph = fract(s.x * gtab.x) + fmaf(s.y, gtab.x, d.z * gtab.y);
+        sincos_turns(ph, &sn, &cs);
+        v[2 * kk] = cs;
+        v[2 * kk + 1] = sn;
       }
-      float sy, cy, sx, cx;
-      const float ty = fract(ph.z * yt.x) + fmaf(ph.w, yt.x, wcv * yt.y);
-      sincos_turns(ty, &sy, &cy);
-      const float tx = fract(ph.x * xt.x) + fmaf(ph.y, xt.x, wcv * xt.y);
-      sincos_turns(tx, &sx, &cx);
-      const float pr = cr * cy - ci * sy;
-      const float pim = cr * sy + ci * cy;
-      pv[2 * kk] = rvalid ? pr : 0.0f;
-      pv[2 * kk + 1] = rvalid ? -pim : 0.0f;
-      qv[2 * kk] = cx;
-      qv[2 * kk + 1] = sx;
     }
     if constexpr (PREC == PREC_F32) {
+      unsigned char* dst = group_b ? sm.b[buf] : sm.a[buf];
 #pragma unroll
-      for (int e = 0; e < kKP; e += 4) {
-        *(float4*)&sm.a[t * S + e] = make_float4(pv[e], pv[e + 1], pv[e + 2], pv[e + 3]);
-        *(float4*)&sm.b[t * S + e] = make_float4(qv[e], qv[e + 1], qv[e + 2], qv[e + 3]);
-      }
+      for (int g = 0; g < NG; g++)
+        *(float4*)&dst[swz<NG>(gidx, g)] =
+            make_float4(v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3]);
     } else {
+      unsigned char* dh = group_b ? sm.bh[buf] : sm.ah[buf];
+      unsigned char* dl = group_b ? sm.bl[buf] : sm.al[buf];
 #pragma unroll
-      for (int e = 0; e < kKP; e += 8) {
-        u16x8_t ph8, pl8, qh8, ql8;
+      for (int g = 0; g < NG; g++) {
+        u16x8_t h8, l8;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
           unsigned short h, l;
-          split16<PREC>(pv[e + q], &h, &l);
-          ph8[q] = h;
-          pl8[q] = l;
-          split16<PREC>(qv[e + q], &h, &l);
-          qh8[q] = h;
-          ql8[q] = l;
+          split16<PREC>(v[8 * g + q], &h, &l);
+          h8[q] = h;
+          l8[q] = l;
         }
-        *(u16x8_t*)&sm.ah[t * S + e] = ph8;
-        *(u16x8_t*)&sm.al[t * S + e] = pl8;
-        *(u16x8_t*)&sm.bh[t * S + e] = qh8;
-        *(u16x8_t*)&sm.bl[t * S + e] = ql8;
+        *(u16x8_t*)&dh[swz<NG>(gidx, g)] = h8;
+        *(u16x8_t*)&dl[swz<NG>(gidx, g)] = l8;
       }
     }
-    __syncthreads();
+  };
 
-    // ---- MFMA over the chunk ----
+  f32x16_t acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[a][b][e] = 0.0f;
+
+  // ---- multiply the chunk in buffer buf ----
+  auto multiply = [&](int buf) {
     if constexpr (PREC == PREC_F32) {
-      // lane (row lr, group lh) takes k = lh*16 + s at step s
-      float av[4][16], bv[4][16];
+      // lane (row lr, group lh) takes k = lh*16 + s at step s: 16-byte groups 4 lh .. 4 lh + 3
 #pragma unroll
-      for (int rb = 0; rb < 4; rb++) {
-        const float* src = &sm.a[(wr + rb * 32 + lr) * S + lh * 16];
-#pragma unroll
-        for (int e = 0; e < 16; e += 4) {
-          const float4 v = *(const float4*)&src[e];
-          av[rb][e] = v.x;
-          av[rb][e + 1] = v.y;
-          av[rb][e + 2] = v.z;
-          av[rb][e + 3] = v.w;
-        }
-      }
-#pragma unroll
-      for (int cb = 0; cb < 4; cb++) {
-        const float* src = &sm.b[(wc + cb * 32 + lr) * S + lh * 16];
-#pragma unroll
-        for (int e = 0; e < 16; e += 4) {
-          const float4 v = *(const float4*)&src[e];
-          bv[cb][e] = v.x;
-          bv[cb][e + 1] = v.y;
-          bv[cb][e + 2] = v.z;
-          bv[cb][e + 3] = v.w;
-        }
-      }
-#pragma unroll
-      for (int s = 0; s < 16; s++)
+      for (int g = 0; g < 4; g++) {
+        float4 av[4], bv[2];
 #pragma unroll
         for (int rb = 0; rb < 4; rb++)
+          av[rb] = *(const float4*)&sm.a[buf][swz<NG>(wr + rb * 32 + lr, 4 * lh + g)];
 #pragma unroll
-          for (int cb = 0; cb < 4; cb++)
-            acc[rb][cb] =
-                __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb][s], bv[cb][s], acc[rb][cb], 0, 0, 0);
+        for (int cb = 0; cb < 2; cb++)
+          bv[cb] = *(const float4*)&sm.b[buf][swz<NG>(wc + cb * 32 + lr, 4 * lh + g)];
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+#pragma unroll
+          for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++)
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[rb][e], bv[cb][e],
+                                                                 acc[rb][cb], 0, 0, 0);
+      }
     } else {
 #pragma unroll
       for (int s = 0; s < kKP / 16; s++) {
-        // lane (row lr, group lh) takes k = s*16 + lh*8 + e
-        u16x8_t ahi[4], alo[4], bhi[4], blo[4];
+        // lane (row lr, group lh) takes k = s*16 + lh*8 + e: 16-byte group 2 s + lh
+        const int g = 2 * s + lh;
+        u16x8_t ahi[4], alo[4], bhi[2], blo[2];
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) {
-          const int o = (wr + rb * 32 + lr) * S + s * 16 + lh * 8;
-          ahi[rb] = *(const u16x8_t*)&sm.ah[o];
-          alo[rb] = *(const u16x8_t*)&sm.al[o];
+          const int o = swz<NG>(wr + rb * 32 + lr, g);
+          ahi[rb] = *(const u16x8_t*)&sm.ah[buf][o];
+          alo[rb] = *(const u16x8_t*)&sm.al[buf][o];
         }
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-          const int o = (wc + cb * 32 + lr) * S + s * 16 + lh * 8;
-          bhi[cb] = *(const u16x8_t*)&sm.bh[o];
-          blo[cb] = *(const u16x8_t*)&sm.bl[o];
+        for (int cb = 0; cb < 2; cb++) {
+          const int o = swz<NG>(wc + cb * 32 + lr, g);
+          bhi[cb] = *(const u16x8_t*)&sm.bh[buf][o];
+          blo[cb] = *(const u16x8_t*)&sm.bl[buf][o];
         }
 #pragma unroll
         for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-          for (int cb = 0; cb < 4; cb++) {
+          for (int cb = 0; cb < 2; cb++) {
             acc[rb][cb] = mfma16(ahi[rb], bhi[cb], acc[rb][cb], PREC);
             acc[rb][cb] = mfma16(ahi[rb], blo[cb], acc[rb][cb], PREC);
             acc[rb][cb] = mfma16(alo[rb], bhi[cb], acc[rb][cb], PREC);
           }
       }
     }
+  };
+
+  // ---- prologue: records of chunks 0 and 1, generate chunk 0 ----
+  if (nchunks > 0) {
+    if (loader) store_rec(0, load_rec(0));
+    if (loader && nchunks > 1) store_rec(1, load_rec(1));
+    __syncthreads();
+    generate(0);
+    __syncthreads();
+  }
+  // ---- main loop: one barrier per chunk ----
+  for (int c = 0; c < nchunks; c++) {
+    const bool more = c + 1 < nchunks;
+    const bool prefetch = loader && c + 2 < nchunks;
+    float4 pre = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (prefetch) pre = load_rec(c + 2);
+    if (!group_b) {
+      if (more) generate((c + 1) & 1);
+      multiply(c & 1);
+    } else {
+      multiply(c & 1);
+      if (more) generate((c + 1) & 1);
+    }
+    if (prefetch) store_rec(c & 1, pre);
     __syncthreads();
   }
 
@@ -285,7 +336,7 @@ __global__ __launch_bounds__(256, 1) void adjoint_gemm_kernel(
 #pragma unroll
   for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-    for (int cb = 0; cb < 4; cb++)
+    for (int cb = 0; cb < 2; cb++)
 #pragma unroll
       for (int e = 0; e < 16; e++) {
         const int r = ti * kTile + wr + rb * 32 + 8 * (e >> 2) + 4 * lh + (e & 3);
@@ -415,15 +466,15 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
     } else {
       const dim3 grid(ntiles * nsplit);
       if (mode == GVM_ADJOINT_F32)
-        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F32>, grid, dim3(256), 0, ctx->stream, s.phs,
+        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F32>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
                            s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
                            ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
       else if (mode == GVM_ADJOINT_F16X3)
-        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F16>, grid, dim3(256), 0, ctx->stream, s.phs,
+        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F16>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
                            s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
                            ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
       else
-        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_BF16>, grid, dim3(256), 0, ctx->stream, s.phs,
+        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_BF16>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
                            s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
                            ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
     }
