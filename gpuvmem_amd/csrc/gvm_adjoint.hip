@@ -92,21 +92,21 @@ __device__ __forceinline__ f32x16_t mfma16(const u16x8_t& a, const u16x8_t& b, f
                                                  __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
 }
 
-// LDS of the GEMM kernel.  Two buffers of P (rows) and Q (columns) tiles,
-// 256 rows x kKP values each; 16-bit precisions keep hi and lo parts in
-// separate arrays.  Rows are addressed in 16-byte groups with an XOR swizzle
-// so that the 32 rows read by one MFMA operand load spread over the banks.
-// Plus two buffers of visibility records for the chunk being generated.
+// LDS of the GEMM kernel: two buffers of P (rows) and Q (columns) tiles,
+// 256 rows x kKP values each (16-bit precisions keep hi and lo parts in
+// separate arrays), addressed in 16-byte groups with an XOR swizzle so the
+// 32 rows read by one MFMA operand load spread over the banks; and two
+// buffers of visibility records.
 template <int PREC>
 struct AdjLds {
-  static constexpr int kRowBytes = 64;           // 32 x 16-bit
+  static constexpr int kRowBytes = 64;            // 32 x 16-bit
   static constexpr int kGroups = kRowBytes / 16;  // 4
   unsigned char ah[2][kTile * kRowBytes];
   unsigned char al[2][kTile * kRowBytes];
   unsigned char bh[2][kTile * kRowBytes];
   unsigned char bl[2][kTile * kRowBytes];
   float4 rec_s[2][kKV];  // (a_hi, a_lo, b_hi, b_lo)
-  float4 rec_d[2][kKV];  // (scale w Re Vr, -scale w Im Vr, w, 0)
+  float4 rec_d[2][kKV];  // (scale |c|, arg c in turns, w, 0)
 };
 
 template <>
@@ -124,26 +124,48 @@ __device__ __forceinline__ int swz(int row, int g) {
   return row * (NG * 16) + ((g ^ ((row >> 2) & (NG - 1))) << 4);
 }
 
+// hi/lo split of a pair of floats into packed 16-bit pairs.  FP16: hi is x
+// truncated to FP16 (RTZ conversion; for normal FP16 values that is x with
+// the low 13 mantissa bits cleared), lo = x - hi exactly in FP32, then
+// converted.  BF16: round to nearest even for both parts.
+template <int PREC>
+__device__ __forceinline__ void split_pair(float x0, float x1, unsigned int* hi, unsigned int* lo) {
+  if constexpr (PREC == PREC_F16) {
+    const float h0 = __uint_as_float(__float_as_uint(x0) & 0xffffe000u);
+    const float h1 = __uint_as_float(__float_as_uint(x1) & 0xffffe000u);
+    *hi = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(x0, x1));
+    *lo = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(x0 - h0, x1 - h1));
+  } else {
+    const unsigned short a = bf16_rne(x0), b = bf16_rne(x1);
+    *hi = (unsigned int)a | ((unsigned int)b << 16);
+    const unsigned short c = bf16_rne(x0 - bf16_to_f(a)), d = bf16_rne(x1 - bf16_to_f(b));
+    *lo = (unsigned int)c | ((unsigned int)d << 16);
+  }
+}
+
 constexpr int kGemmThreads = 512;
+constexpr int kHalfKV = kKV / 2;
 
 // One workgroup (8 waves, 2 per SIMD) owns a 256x256 tile of G for one
 // K-range of visibilities.  Wave w accumulates rows (w&1)*128.. and columns
 // (w>>1)*64.. as 4x2 blocks of 32x32 MFMA outputs (128 FP32 accumulators per
-// lane).  Per chunk of kKV visibilities:
-//   threads   0..255 (waves 0-3) generate P row t   (c Ey, with the chi2 weight),
-//   threads 256..511 (waves 4-7) generate Q column t (Ex),
-// into the other LDS buffer while the chunk before is multiplied.  Waves 0-3
-// generate first and multiply second, waves 4-7 the other way round, so on
-// each SIMD one wave issues VALU work while the other keeps the MFMA pipe
-// busy; one barrier per chunk.  Visibility records are loaded from HBM two
-// chunks ahead by 32 lanes of wave 0 and staged in LDS.
+// lane).  Per chunk of kKV visibilities, thread t generates P (row t%256) and
+// Q (column t%256) for half of the chunk's visibilities (t/256 picks the
+// half), into the other LDS buffer while the chunk before is multiplied.
+// Waves 0-3 generate first and multiply second, waves 4-7 the other way
+// round, so on each SIMD one wave issues VALU work while the other keeps the
+// MFMA pipe busy; one barrier per chunk.  Visibility records are loaded from
+// HBM two chunks ahead by wave 0 and staged in LDS.
+//
+// P uses c in polar form (from the forward pass): c Ey = |c| e^{2 pi i
+// (phi_y + arg c)}, so the residual enters as one extra phase term and one
+// scale per value.
 template <int PREC>
 __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
     const float4* __restrict__ phs, const float* __restrict__ wcoord,
-    const float* __restrict__ weight, const float2* __restrict__ vr, long nvis,
-    const float2* __restrict__ xtab, const float2* __restrict__ ytab, int N, int tiles_x,
-    int ntiles, int nsplit, long kper, const unsigned int* __restrict__ cmax,
-    float* __restrict__ part) {
+    const float2* __restrict__ cpol, long nvis, const float2* __restrict__ xtab,
+    const float2* __restrict__ ytab, int N, int tiles_x, int ntiles, int nsplit, long kper,
+    const unsigned int* __restrict__ cmax, float* __restrict__ part) {
   __shared__ AdjLds<PREC> sm;
   constexpr int NG = AdjLds<PREC>::kGroups;
 
@@ -162,85 +184,76 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const bool group_b = wave >= 4;  // Q generators; multiply first
+  const bool group_b = wave >= 4;  // multiply first
   const int gidx = t & (kTile - 1);
-  // generator constants: row (P) or column (Q) of this thread
-  bool gvalid;
-  float2 gtab;
-  if (!group_b) {
-    const int row = ti * kTile + gidx;
-    gvalid = row < N;
-    gtab = gvalid ? ytab[row] : make_float2(0.f, 0.f);
-  } else {
-    const int col = tj * kTile + gidx;
-    gvalid = col < N;
-    gtab = gvalid ? xtab[col] : make_float2(0.f, 0.f);
-  }
+  const int half = t >> 8;  // which half of the chunk's visibilities this thread generates
+  const int row = ti * kTile + gidx, col = tj * kTile + gidx;
+  const bool rvalid = row < N, cvalid = col < N;
+  const float2 yt = rvalid ? ytab[row] : make_float2(0.f, 0.f);
+  const float2 xt = cvalid ? xtab[col] : make_float2(0.f, 0.f);
 
   const float cm = __uint_as_float(*cmax);
   const float scale = cm > 0.0f ? exp2f(-ceilf(log2f(cm))) : 1.0f;
+  const float rscale = rvalid ? scale : 0.0f;  // P of rows past the image is 0
 
   const int lr = lane & 31, lh = lane >> 5;
   const int wr = (wave & 1) * 128, wc = (wave >> 1) * 64;
 
-  // ---- visibility records (wave 0, lanes 0..31) ----
-  const bool loader = t < 2 * kKV;
-  auto load_rec = [&](int c) -> float4 {
-    const long k = kbeg + (long)c * kKV + (lane & (kKV - 1));
-    if (k >= kend) return make_float4(0.f, 0.f, 0.f, 0.f);
-    if (lane < kKV) return phs[k];
-    const float2 r = vr[k];
-    const float w = weight[k] * scale;
-    return make_float4(w * r.x, -w * r.y, wcoord[k], 0.f);
+  // ---- visibility records (wave 0; lane l handles slot l % kKV) ----
+  const int slot = lane & (kKV - 1);
+  auto load_rec = [&](int c, float4& s, float4& d) {
+    const long k = kbeg + (long)c * kKV + slot;
+    const long kc = min(k, nvis - 1);
+    s = phs[kc];
+    const float2 p = cpol[kc];
+    d = make_float4(k < kend ? p.x : 0.0f, p.y, wcoord[kc], 0.f);
   };
-  auto store_rec = [&](int buf, float4 v) {
-    if (lane < kKV)
-      sm.rec_s[buf][lane] = v;
-    else
-      sm.rec_d[buf][lane - kKV] = v;
+  auto store_rec = [&](int buf, const float4& s, const float4& d) {
+    sm.rec_s[buf][slot] = s;
+    sm.rec_d[buf][slot] = d;
   };
 
-  // ---- P / Q generation for chunk c into buffer buf ----
+  // ---- generation: P row gidx and Q column gidx for half of the chunk ----
   auto generate = [&](int buf) {
-    float v[kKP];
+    float pv[2 * kHalfKV], qv[2 * kHalfKV];
 #pragma unroll
-    for (int kk = 0; kk < kKV; kk++) {
+    for (int q = 0; q < kHalfKV; q++) {
+      const int kk = half * kHalfKV + q;
       const float4 s = sm.rec_s[buf][kk];
       const float4 d = sm.rec_d[buf][kk];
-      float sn, cs;
-      if (!group_b) {
-        const float ph = fract(s.z * gtab.x) + fmaf(s.w, gtab.x, d.z * gtab.y);
-        sincos_turns(ph, &sn, &cs);
-        v[2 * kk] = gvalid ? d.x * cs - d.y * sn : 0.0f;
-        v[2 * kk + 1] = gvalid ? -(d.x * sn + d.y * cs) : 0.0f;
-      } else {
-        const float ph = fract(s.x * gtab.x) + fmaf(s.y, gtab.x, d.z * gtab.y);
-        sincos_turns(ph, &sn, &cs);
-        v[2 * kk] = cs;
-        v[2 * kk + 1] = sn;
-      }
+      float sy, cy, sx, cx;
+      sincos_turns(fract(s.z * yt.x) + fmaf(s.w, yt.x, fmaf(d.z, yt.y, d.y)), &sy, &cy);
+      sincos_turns(fract(s.x * xt.x) + fmaf(s.y, xt.x, d.z * xt.y), &sx, &cx);
+      const float amp = d.x * rscale;
+      pv[2 * q] = amp * cy;
+      pv[2 * q + 1] = -(amp * sy);
+      qv[2 * q] = cx;
+      qv[2 * q + 1] = sx;
     }
     if constexpr (PREC == PREC_F32) {
-      unsigned char* dst = group_b ? sm.b[buf] : sm.a[buf];
 #pragma unroll
-      for (int g = 0; g < NG; g++)
-        *(float4*)&dst[swz<NG>(gidx, g)] =
-            make_float4(v[4 * g], v[4 * g + 1], v[4 * g + 2], v[4 * g + 3]);
+      for (int g = 0; g < 4; g++) {
+        const int o = swz<NG>(gidx, 4 * half + g);
+        *(float4*)&sm.a[buf][o] = make_float4(pv[4 * g], pv[4 * g + 1], pv[4 * g + 2], pv[4 * g + 3]);
+        *(float4*)&sm.b[buf][o] = make_float4(qv[4 * g], qv[4 * g + 1], qv[4 * g + 2], qv[4 * g + 3]);
+      }
     } else {
-      unsigned char* dh = group_b ? sm.bh[buf] : sm.ah[buf];
-      unsigned char* dl = group_b ? sm.bl[buf] : sm.al[buf];
 #pragma unroll
-      for (int g = 0; g < NG; g++) {
-        u16x8_t h8, l8;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          unsigned short h, l;
-          split16<PREC>(v[8 * g + q], &h, &l);
-          h8[q] = h;
-          l8[q] = l;
-        }
-        *(u16x8_t*)&dh[swz<NG>(gidx, g)] = h8;
-        *(u16x8_t*)&dl[swz<NG>(gidx, g)] = l8;
+      for (int g = 0; g < 2; g++) {
+        uint4 ph, pl, qh, ql;
+        split_pair<PREC>(pv[8 * g + 0], pv[8 * g + 1], &ph.x, &pl.x);
+        split_pair<PREC>(pv[8 * g + 2], pv[8 * g + 3], &ph.y, &pl.y);
+        split_pair<PREC>(pv[8 * g + 4], pv[8 * g + 5], &ph.z, &pl.z);
+        split_pair<PREC>(pv[8 * g + 6], pv[8 * g + 7], &ph.w, &pl.w);
+        split_pair<PREC>(qv[8 * g + 0], qv[8 * g + 1], &qh.x, &ql.x);
+        split_pair<PREC>(qv[8 * g + 2], qv[8 * g + 3], &qh.y, &ql.y);
+        split_pair<PREC>(qv[8 * g + 4], qv[8 * g + 5], &qh.z, &ql.z);
+        split_pair<PREC>(qv[8 * g + 6], qv[8 * g + 7], &qh.w, &ql.w);
+        const int o = swz<NG>(gidx, 2 * half + g);
+        *(uint4*)&sm.ah[buf][o] = ph;
+        *(uint4*)&sm.al[buf][o] = pl;
+        *(uint4*)&sm.bh[buf][o] = qh;
+        *(uint4*)&sm.bl[buf][o] = ql;
       }
     }
   };
@@ -305,29 +318,36 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
     }
   };
 
-  // ---- prologue: records of chunks 0 and 1, generate chunk 0 ----
   if (nchunks > 0) {
-    if (loader) store_rec(0, load_rec(0));
-    if (loader && nchunks > 1) store_rec(1, load_rec(1));
+    // prologue: records of chunks 0 and 1, tiles of chunk 0
+    float4 sn, dn;
+    if (wave == 0) {
+      float4 s1, d1;
+      load_rec(0, sn, dn);
+      load_rec(1, s1, d1);
+      store_rec(0, sn, dn);
+      store_rec(1, s1, d1);
+      load_rec(2, sn, dn);
+    }
     __syncthreads();
     generate(0);
     __syncthreads();
-  }
-  // ---- main loop: one barrier per chunk ----
-  for (int c = 0; c < nchunks; c++) {
-    const bool more = c + 1 < nchunks;
-    const bool prefetch = loader && c + 2 < nchunks;
-    float4 pre = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (prefetch) pre = load_rec(c + 2);
-    if (!group_b) {
-      if (more) generate((c + 1) & 1);
-      multiply(c & 1);
-    } else {
-      multiply(c & 1);
-      if (more) generate((c + 1) & 1);
+    // main loop: one barrier per chunk
+    for (int c = 0; c < nchunks; c++) {
+      const int cur = c & 1, nxt = cur ^ 1;
+      if (!group_b) {
+        generate(nxt);  // chunk c+1 (past the end: zero weights, never multiplied)
+        multiply(cur);
+      } else {
+        multiply(cur);
+        generate(nxt);
+      }
+      if (wave == 0) {
+        store_rec(cur, sn, dn);  // records of chunk c+2
+        load_rec(c + 3, sn, dn);
+      }
+      __syncthreads();
     }
-    if (prefetch) store_rec(c & 1, pre);
-    __syncthreads();
   }
 
   // ---- write the partial tile: 32x32 accumulator layout, lane l, reg e:
@@ -467,15 +487,15 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
       const dim3 grid(ntiles * nsplit);
       if (mode == GVM_ADJOINT_F32)
         hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F32>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
-                           s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
+                           s.wcoord, s.cpol, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
                            ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
       else if (mode == GVM_ADJOINT_F16X3)
         hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F16>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
-                           s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
+                           s.wcoord, s.cpol, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
                            ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
       else
         hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_BF16>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
-                           s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
+                           s.wcoord, s.cpol, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
                            ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
     }
     GVM_CHECK_LAUNCH();

@@ -197,6 +197,7 @@ static void free_slice(Slice& s) {
   (void)hipFree(s.wcoord);
   (void)hipFree(s.pb);
   (void)hipFree(s.vr);
+  (void)hipFree(s.cpol);
   s = Slice{};
 }
 
@@ -403,11 +404,13 @@ extern "C" gvm_status gvm_add_visibilities(gvm_ctx* ctx, float freq, long nvis,
   if ((st = upload(&s.cell, cell)) || (st = upload(&s.frac, frac)) ||
       (st = upload(&s.vo, vov)) || (st = upload(&s.weight, wv)) ||
       (st = upload(&s.phs, phs)) || (st = upload(&s.wcoord, wc)) ||
-      (st = upload(&s.pb, pb)) || (st = dev_alloc(&s.vr, nvis))) {
+      (st = upload(&s.pb, pb)) || (st = dev_alloc(&s.vr, nvis)) ||
+      (st = dev_alloc(&s.cpol, nvis))) {
     free_slice(s);
     return st;
   }
   GVM_HIP_TRY(hipMemset(s.vr, 0, sizeof(float2) * nvis));
+  GVM_HIP_TRY(hipMemset(s.cpol, 0, sizeof(float2) * nvis));
   GVM_HIP_TRY(hipDeviceSynchronize());
   ctx->slices.push_back(s);
   ctx->have_residuals = false;

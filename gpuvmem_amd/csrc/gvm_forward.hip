@@ -227,13 +227,13 @@ gvm_status model_grid(gvm_ctx* ctx, const float* I, const Slice& s) {
 // ---------------------------------------------------------------------------
 // vis_mod + residual + chi2Vector + reduction:
 // src/functions.cu:2557-2609, 2663-2671, 2867-2876.  Bytes per visibility:
-// cell 4 + frac 8 + Vo 8 + weight 4 read, Vr 8 written.
+// cell 4 + frac 8 + Vo 8 + weight 4 read, Vr 8 + polar c 8 written.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void degrid_chi2_kernel(
     const float2* __restrict__ grid, int N, const int* __restrict__ cell,
     const float2* __restrict__ frac, const float2* __restrict__ vo,
-    const float* __restrict__ weight, float2* __restrict__ vr, long nvis,
-    double* __restrict__ partials, unsigned int* __restrict__ cmax) {
+    const float* __restrict__ weight, float2* __restrict__ vr, float2* __restrict__ cpol,
+    long nvis, double* __restrict__ partials, unsigned int* __restrict__ cmax) {
   __shared__ double sh[4];
   double acc = 0.0;
   float cm = 0.0f;
@@ -263,7 +263,10 @@ __global__ __launch_bounds__(256) void degrid_chi2_kernel(
     const float w = weight[v];
     const float a2 = r.x * r.x + r.y * r.y;
     acc += (double)(w * a2);
-    cm = fmaxf(cm, w * sqrtf(a2));
+    const float amp = w * sqrtf(a2);
+    cm = fmaxf(cm, amp);
+    // c = w conj(Vr) in polar form for the adjoint: (|c|, arg(c) in turns)
+    cpol[v] = make_float2(amp, -atan2f(r.y, r.x) * 0.15915494309189535f);
   }
   const double tot = block_sum(acc, sh);
   __syncthreads();
@@ -291,7 +294,7 @@ gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize, dou
     gvm_status st = model_grid(ctx, I, s);
     if (st) return st;
     hipLaunchKernelGGL(degrid_chi2_kernel, dim3(kReduceBlocks), dim3(256), 0, ctx->stream,
-                       ctx->grid, g.N, s.cell, s.frac, s.vo, s.weight, s.vr, s.nvis,
+                       ctx->grid, g.N, s.cell, s.frac, s.vo, s.weight, s.vr, s.cpol, s.nvis,
                        ctx->partials + (long)si * kReduceBlocks, ctx->cmax + si);
     GVM_CHECK_LAUNCH();
   }

@@ -48,6 +48,7 @@ struct Slice {
   float* pb = nullptr;       // primary beam at this frequency, float[M*N]
   // per-evaluation data
   float2* vr = nullptr;      // residuals Vo - Vm
+  float2* cpol = nullptr;    // (w |Vr|, -arg(Vr) / 2 pi): c = w conj(Vr) in polar form, turns
 };
 
 constexpr int kReduceBlocks = 1024;
