@@ -156,3 +156,25 @@ extern "C" gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimiz
   }
   return GVM_OK;
 }
+
+extern "C" int gvm_fw_registered(const char* kind, const char* name) {
+  if (!kind || !name) return 0;
+  const std::string k(kind), n(name);
+  try {
+    if (k == "Fi") {
+      std::unique_ptr<Fi> p(createObject<Fi, std::string>(n));
+      return p ? 1 : 0;
+    }
+    if (k == "Optimizer") {
+      std::unique_ptr<Optimizer> p(createObject<Optimizer, std::string>(n));
+      return p ? 1 : 0;
+    }
+    if (k == "ObjectiveFunction") {
+      std::unique_ptr<ObjectiveFunction> p(createObject<ObjectiveFunction, std::string>(n));
+      return p ? 1 : 0;
+    }
+  } catch (const gvm::FactoryError&) {
+    return 0;
+  }
+  return 0;
+}

@@ -149,7 +149,7 @@ def make_problem(which: int = 0, noise_cut_factor: float = 10.0, **overrides) ->
 
 
 def small_problem(npix: int = 64, n_antennas: int = 8, seed: int = 0, **overrides) -> Problem:
-    """A problem small enough for the oracle's direct sums (tests)."""
+    """A problem small enough for FP64 direct sums in tests."""
     kw = dict(npix=npix, n_antennas=n_antennas, ha_start_hours=-0.5, ha_end_hours=0.5,
               dump_seconds=60.0, cell_arcsec=0.05, n_gaussians=3, n_points=2,
               position_radius_frac=0.25, seed_layout=seed, seed_sky=seed + 1,

@@ -68,6 +68,10 @@ gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimizer, int tot
 
 const char* gvm_fw_last_error(void);
 
+/* 1 if `name` is registered with the factory of `kind` ("Fi", "Optimizer",
+ * "ObjectiveFunction"), else 0. */
+int gvm_fw_registered(const char* kind, const char* name);
+
 #ifdef __cplusplus
 }
 #endif

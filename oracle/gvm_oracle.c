@@ -724,6 +724,24 @@ void orc_of_calc_gradient(orc_objective* of, double* p, double* xi, int iter) {
   memcpy(xi, of->dphi, sizeof(double) * tot);
 }
 
+static double of_value_cb(void* ctx, double* p) {
+  return orc_of_calc_function((orc_objective*)ctx, p);
+}
+static void of_gradient_cb(void* ctx, double* p, double* xi, int iter) {
+  orc_of_calc_gradient((orc_objective*)ctx, p, xi, iter);
+}
+orc_problem orc_of_problem(orc_objective* of) {
+  orc_problem pr;
+  pr.n = (long)of->g->M * of->g->N;
+  pr.image_count = of->image_count;
+  pr.nopositivity = of->nopositivity;
+  pr.floor_v = -1.0 * (double)of->g->eta * (double)of->g->minpix;
+  pr.value = of_value_cb;
+  pr.gradient = of_gradient_cb;
+  pr.ctx = of;
+  return pr;
+}
+
 /* ------------------------------------------------------------------------
  * Line search (row a18).  Bracketing by golden-ratio expansion with
  * parabolic extrapolation, then Brent's parabolic/golden-section
@@ -847,22 +865,24 @@ double orc_brent(double ax, double bx, double cx, double tol, double* xmin,
  * x_t = pcom + x xicom; image 0 is floored at -eta*MINPIX (positivity), the
  * other images are not (unless nopositivity, where none is). */
 typedef struct {
-  orc_objective* of;
+  const orc_problem* pr;
   const double* pcom;
   const double* xicom;
   double* xt;
+  long calls;
 } f1dim_ctx;
 
 static double f1dim(double x, void* vctx) {
   f1dim_ctx* c = (f1dim_ctx*)vctx;
-  orc_objective* of = c->of;
-  long n = (long)of->g->M * of->g->N;
-  double floor_v = -1.0 * (double)of->g->eta * (double)of->g->minpix;
-  for (int im = 0; im < of->image_count; im++) {
+  const orc_problem* pr = c->pr;
+  long n = pr->n;
+  double floor_v = pr->floor_v;
+  c->calls++;
+  for (int im = 0; im < pr->image_count; im++) {
     const double* pc = c->pcom + n * im;
     const double* xc = c->xicom + n * im;
     double* xt = c->xt + n * im;
-    if (im == 0 && !of->nopositivity) {
+    if (im == 0 && !pr->nopositivity) {
       for (long k = 0; k < n; k++) {
         double t = pc[k] + x * xc[k];
         xt[k] = t > floor_v ? t : floor_v;
@@ -871,29 +891,32 @@ static double f1dim(double x, void* vctx) {
       for (long k = 0; k < n; k++) xt[k] = pc[k] + x * xc[k];
     }
   }
-  return orc_of_calc_function(of, c->xt);
+  return pr->value(pr->ctx, c->xt);
 }
 
 /* src/linmin.cu:52-110 and the newP kernels src/functions.cu:2779-2812:
  * bracket from (0, 1), Brent with TOL 1e-7, then xi <- xmin xi and
  * p <- p + xi with the image-0 floor (xi set to 0 where floored). */
-double orc_linmin(orc_objective* of, double* p, double* xi) {
-  long n = (long)of->g->M * of->g->N;
-  long tot = n * of->image_count;
+static long g_linmin_calls = 0;
+
+double orc_linmin(const orc_problem* pr, double* p, double* xi) {
+  long n = pr->n;
+  long tot = n * pr->image_count;
   double* pcom = (double*)malloc(sizeof(double) * tot);
   double* xicom = (double*)malloc(sizeof(double) * tot);
   double* xt = (double*)malloc(sizeof(double) * tot);
   memcpy(pcom, p, sizeof(double) * tot);
   memcpy(xicom, xi, sizeof(double) * tot);
-  f1dim_ctx c = {of, pcom, xicom, xt};
+  f1dim_ctx c = {pr, pcom, xicom, xt, 0};
   double ax = 0.0, xx = 1.0, bx, fa, fx, fb, xmin;
   orc_mnbrak(&ax, &xx, &bx, &fa, &fx, &fb, f1dim, &c);
   double fret = orc_brent(ax, xx, bx, 1.0e-7, &xmin, f1dim, &c);
-  double floor_v = -1.0 * (double)of->g->eta * (double)of->g->minpix;
-  for (int im = 0; im < of->image_count; im++) {
+  g_linmin_calls += c.calls;
+  double floor_v = pr->floor_v;
+  for (int im = 0; im < pr->image_count; im++) {
     double* pp = p + n * im;
     double* xx2 = xi + n * im;
-    if (im == 0 && !of->nopositivity) {
+    if (im == 0 && !pr->nopositivity) {
       for (long k = 0; k < n; k++) {
         xx2[k] *= xmin;
         if (pp[k] + xx2[k] > floor_v) {
@@ -922,16 +945,18 @@ double orc_linmin(orc_objective* of, double* p, double* xi) {
  * (src/functions.cu:3599-3612, 2729-2749, 3655-3687).
  * ---------------------------------------------------------------------- */
 #define OPT_EPS 1.0e-10
-int orc_frprmn(orc_objective* of, double* p, int total_iterations, double ftol,
-               double gtol, orc_opt_result* res) {
-  long n = (long)of->g->M * of->g->N;
-  long tot = n * of->image_count;
+int orc_frprmn(const orc_problem* pr, double* p, int total_iterations, double ftol,
+               double gtol, orc_opt_result* res, double* traj, int cap) {
+  long tot = pr->n * pr->image_count;
+  long ng = 0, nv0 = g_linmin_calls;
   double* g = (double*)calloc(tot, sizeof(double));
   double* h = (double*)calloc(tot, sizeof(double));
   double* xi = (double*)calloc(tot, sizeof(double));
   memset(res, 0, sizeof(*res));
-  double fp = orc_of_calc_function(of, p);
-  orc_of_calc_gradient(of, p, xi, 0);
+  long nv = 1;
+  double fp = pr->value(pr->ctx, p);
+  pr->gradient(pr->ctx, p, xi, 0);
+  ng++;
   for (long k = 0; k < tot; k++) {
     g[k] = -xi[k];
     xi[k] = h[k] = g[k];
@@ -939,13 +964,16 @@ int orc_frprmn(orc_objective* of, double* p, int total_iterations, double ftol,
   int reason = 0, it;
   double fret = fp;
   for (it = 1; it <= total_iterations; it++) {
-    fret = orc_linmin(of, p, xi);
+    fret = orc_linmin(pr, p, xi);
+    if (traj && it - 1 < cap) traj[it - 1] = fret;
     if (2.0 * fabs(fret - fp) <= ftol * (fabs(fret) + fabs(fp) + OPT_EPS)) {
       reason = 1;
       break;
     }
-    fp = orc_of_calc_function(of, p);
-    orc_of_calc_gradient(of, p, xi, it);
+    fp = pr->value(pr->ctx, p);
+    nv++;
+    pr->gradient(pr->ctx, p, xi, it);
+    ng++;
     double den = fp > 1.0 ? fp : 1.0;
     double gmax = 0.0;
     for (long k = 0; k < tot; k++) {
@@ -974,9 +1002,11 @@ int orc_frprmn(orc_objective* of, double* p, int total_iterations, double ftol,
   res->iterations = it > total_iterations ? total_iterations : it;
   res->exit_reason = reason;
   res->fret = fret;
-  res->final_value = orc_of_calc_function(of, p);
-  res->n_fcalls = of->n_fcalls;
-  res->n_gcalls = of->n_gcalls;
+  res->final_value = pr->value(pr->ctx, p);
+  res->n_values = nv + 1 + (g_linmin_calls - nv0);
+  res->n_gradients = ng;
+  res->n_fcalls = res->n_values;
+  res->n_gcalls = ng;
   free(g);
   free(h);
   free(xi);
@@ -998,10 +1028,10 @@ static double dotn(const double* a, const double* b, long n) {
   return s;
 }
 
-static void lbfgs_recursion(orc_objective* of, double* d_y, double* d_s,
+static void lbfgs_recursion(const orc_problem* pr, double* d_y, double* d_s,
                             double* xi, int par_M, int lbfgs_it) {
-  long n = (long)of->g->M * of->g->N;
-  int ic = of->image_count;
+  long n = pr->n;
+  int ic = pr->image_count;
   long tot = n * ic;
   double* q = (double*)malloc(sizeof(double) * tot);
   double* r = (double*)calloc(tot, sizeof(double));
@@ -1047,10 +1077,11 @@ static void lbfgs_recursion(orc_objective* of, double* d_y, double* d_s,
   free(alpha);
 }
 
-int orc_lbfgs(orc_objective* of, double* p, int K, int total_iterations,
-              double ftol, double gtol, orc_opt_result* res) {
-  long n = (long)of->g->M * of->g->N;
-  int ic = of->image_count;
+int orc_lbfgs(const orc_problem* pr, double* p, int K, int total_iterations,
+              double ftol, double gtol, orc_opt_result* res, double* traj, int cap) {
+  long n = pr->n;
+  int ic = pr->image_count;
+  long ng = 0, nv0 = g_linmin_calls, nv = 0;
   long tot = n * ic;
   double* d_y = (double*)calloc((size_t)tot * K, sizeof(double));
   double* d_s = (double*)calloc((size_t)tot * K, sizeof(double));
@@ -1058,15 +1089,18 @@ int orc_lbfgs(orc_objective* of, double* p, int K, int total_iterations,
   double* xi = (double*)calloc(tot, sizeof(double));
   double* xi_old = (double*)calloc(tot, sizeof(double));
   memset(res, 0, sizeof(*res));
-  double fp = orc_of_calc_function(of, p);
-  orc_of_calc_gradient(of, p, xi, 0);
+  double fp = pr->value(pr->ctx, p);
+  nv++;
+  pr->gradient(pr->ctx, p, xi, 0);
+  ng++;
   for (long k = 0; k < tot; k++) xi[k] = -xi[k];
   int reason = 0, it;
   double fret = fp;
   for (it = 1; it <= total_iterations; it++) {
     memcpy(p_old, p, sizeof(double) * tot);
     memcpy(xi_old, xi, sizeof(double) * tot);
-    fret = orc_linmin(of, p, xi);
+    fret = orc_linmin(pr, p, xi);
+    if (traj && it - 1 < cap) traj[it - 1] = fret;
     double den = fmax(fmax(fabs(fret), fabs(fp)), 1.0);
     if ((fp - fret) / den <= ftol) {
       reason = 1;
@@ -1079,8 +1113,10 @@ int orc_lbfgs(orc_objective* of, double* p, int K, int total_iterations,
       reason = 2;
       break;
     }
-    fp = orc_of_calc_function(of, p);
-    orc_of_calc_gradient(of, p, xi, it);
+    fp = pr->value(pr->ctx, p);
+    nv++;
+    pr->gradient(pr->ctx, p, xi, it);
+    ng++;
     int slot = (it - 1) % K;
     for (int im = 0; im < ic; im++) {
       double* y = d_y + n * ((long)im * slot + im);
@@ -1090,14 +1126,16 @@ int orc_lbfgs(orc_objective* of, double* p, int K, int total_iterations,
         s[k] = p[n * im + k] - p_old[n * im + k];
       }
     }
-    lbfgs_recursion(of, d_y, d_s, xi, K < it ? K : it, slot);
+    lbfgs_recursion(pr, d_y, d_s, xi, K < it ? K : it, slot);
   }
   res->iterations = it > total_iterations ? total_iterations : it;
   res->exit_reason = reason;
   res->fret = fret;
-  res->final_value = orc_of_calc_function(of, p);
-  res->n_fcalls = of->n_fcalls;
-  res->n_gcalls = of->n_gcalls;
+  res->final_value = pr->value(pr->ctx, p);
+  res->n_values = nv + 1 + (g_linmin_calls - nv0);
+  res->n_gradients = ng;
+  res->n_fcalls = res->n_values;
+  res->n_gcalls = ng;
   free(d_y);
   free(d_s);
   free(p_old);

@@ -154,24 +154,41 @@ void orc_of_calc_gradient(orc_objective* of, double* p, double* xi, int iter);
 
 /* ---- line search + optimisers (rows a16-a18) ---- */
 typedef double (*orc_f1)(double x, void* ctx);
+
+/* A generic objective for the optimisers: image_count images of n values;
+ * image 0 is floored at floor_v during the line search unless nopositivity. */
+typedef struct {
+  long n;
+  int image_count;
+  int nopositivity;
+  double floor_v;
+  double (*value)(void* ctx, double* p);
+  void (*gradient)(void* ctx, double* p, double* xi, int iter);
+  void* ctx;
+} orc_problem;
+
+/* The imaging objective as an orc_problem (ctx = the orc_objective). */
+orc_problem orc_of_problem(orc_objective* of);
 void orc_mnbrak(double* ax, double* bx, double* cx, double* fa, double* fb,
                 double* fc, orc_f1 f, void* ctx);
 double orc_brent(double ax, double bx, double cx, double tol, double* xmin,
                  orc_f1 f, void* ctx);
-double orc_linmin(orc_objective* of, double* p, double* xi);
+double orc_linmin(const orc_problem* pr, double* p, double* xi);
 
 typedef struct {
   int iterations;  /* iterations completed */
+  long n_values, n_gradients;
   int exit_reason; /* 0 iter limit, 1 ftol, 2 gtol, 3 gg==0 */
   double fret;
   double final_value;
   long n_fcalls, n_gcalls;
 } orc_opt_result;
 
-int orc_frprmn(orc_objective* of, double* p, int total_iterations, double ftol,
-               double gtol, orc_opt_result* res);
-int orc_lbfgs(orc_objective* of, double* p, int K, int total_iterations,
-              double ftol, double gtol, orc_opt_result* res);
+/* traj (optional, capacity cap) receives the value after each line search. */
+int orc_frprmn(const orc_problem* pr, double* p, int total_iterations, double ftol,
+               double gtol, orc_opt_result* res, double* traj, int cap);
+int orc_lbfgs(const orc_problem* pr, double* p, int K, int total_iterations,
+              double ftol, double gtol, orc_opt_result* res, double* traj, int cap);
 
 /* ---- CPU gridding comparator (reference src/functions.cu:1339-1653) ---- */
 void orc_do_gridding_pillbox(int M, int N, double deltau, double deltav,

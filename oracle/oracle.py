@@ -62,11 +62,20 @@ class OObjective(C.Structure):
 
 
 class OResult(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("exit_reason", C.c_int), ("fret", C.c_double),
+    _fields_ = [("iterations", C.c_int), ("n_values", C.c_long), ("n_gradients", C.c_long),
+                ("exit_reason", C.c_int), ("fret", C.c_double),
                 ("final_value", C.c_double), ("n_fcalls", C.c_long), ("n_gcalls", C.c_long)]
 
 
 F1 = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+VALUE_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.POINTER(C.c_double))
+GRAD_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int)
+
+
+class OProblem(C.Structure):
+    _fields_ = [("n", C.c_long), ("image_count", C.c_int), ("nopositivity", C.c_int),
+                ("floor_v", C.c_double), ("value", C.c_void_p), ("gradient", C.c_void_p),
+                ("ctx", C.c_void_p)]
 FI_KIND = {"Chi2": 0, "Entropy": 1, "TotalSquaredVariation": 2, "Laplacian": 3}
 
 _lib = None
@@ -112,9 +121,10 @@ def lib() -> C.CDLL:
             "orc_of_calc_gradient": ([C.POINTER(OObjective), P, P, I], None),
             "orc_mnbrak": ([P, P, P, P, P, P, F1, P], None),
             "orc_brent": ([D, D, D, D, P, F1, P], D),
-            "orc_linmin": ([C.POINTER(OObjective), P, P], D),
-            "orc_frprmn": ([C.POINTER(OObjective), P, I, D, D, C.POINTER(OResult)], I),
-            "orc_lbfgs": ([C.POINTER(OObjective), P, I, I, D, D, C.POINTER(OResult)], I),
+            "orc_of_problem": ([C.POINTER(OObjective)], OProblem),
+            "orc_linmin": ([C.POINTER(OProblem), P, P], D),
+            "orc_frprmn": ([C.POINTER(OProblem), P, I, D, D, C.POINTER(OResult), P, I], I),
+            "orc_lbfgs": ([C.POINTER(OProblem), P, I, I, D, D, C.POINTER(OResult), P, I], I),
             "orc_do_gridding_pillbox": ([I, I, D, D, CP, I, I, I, P, P, P], None),
         }
         for name, (args, res) in sig.items():
@@ -300,14 +310,55 @@ class Oracle:
 
     def optimize(self, of, image, optimizer: str = "CG-FRPRMN", iterations: int = 50,
                  ftol: float = 1e-12, gtol: float = 1e-12, lbfgs_k: int = 100):
+        """Returns (final image, OResult, values after each line search)."""
         img = np.ascontiguousarray(image, np.float64).copy()
-        res = OResult()
-        if optimizer == "CG-FRPRMN":
-            self._lib.orc_frprmn(C.byref(of), _ptr(img), iterations, ftol, gtol, C.byref(res))
-        else:
-            self._lib.orc_lbfgs(C.byref(of), _ptr(img), lbfgs_k, iterations, ftol, gtol,
-                                C.byref(res))
-        return img, res
+        pr = self._lib.orc_of_problem(C.byref(of))
+        return run_optimizer(pr, img, optimizer, iterations, ftol, gtol, lbfgs_k)
+
+
+def run_optimizer(pr: OProblem, img: np.ndarray, optimizer: str, iterations: int,
+                  ftol: float = 1e-12, gtol: float = 1e-12, lbfgs_k: int = 100):
+    """Runs frprmn ("CG-FRPRMN") or lbfgs ("CG-LBFGS") on img in place."""
+    res = OResult()
+    traj = np.zeros(max(1, iterations), np.float64)
+    L = lib()
+    if optimizer == "CG-FRPRMN":
+        L.orc_frprmn(C.byref(pr), _ptr(img), iterations, ftol, gtol, C.byref(res), _ptr(traj),
+                     iterations)
+    elif optimizer == "CG-LBFGS":
+        L.orc_lbfgs(C.byref(pr), _ptr(img), lbfgs_k, iterations, ftol, gtol, C.byref(res),
+                    _ptr(traj), iterations)
+    else:
+        raise KeyError(optimizer)
+    return img, res, traj[: min(res.iterations, iterations)].copy()
+
+
+class PyProblem:
+    """An oracle optimiser problem defined by Python callbacks (tests):
+    value(p) -> float and gradient(p, it) -> array, on image_count images of
+    n values (numpy views of the oracle's buffers)."""
+
+    def __init__(self, n, image_count, value, gradient, nopositivity=True, floor_v=0.0):
+        self.n, self.ic = n, image_count
+        self._value, self._gradient = value, gradient
+        tot = n * image_count
+
+        def v(_, p):
+            return float(self._value(np.ctypeslib.as_array(p, (tot,))))
+
+        def g(_, p, xi, it):
+            out = np.ctypeslib.as_array(xi, (tot,))
+            out[:] = self._gradient(np.ctypeslib.as_array(p, (tot,)), it)
+
+        self._v = VALUE_FN(v)
+        self._g = GRAD_FN(g)
+        self.problem = OProblem(n, image_count, int(nopositivity), floor_v,
+                                C.cast(self._v, C.c_void_p), C.cast(self._g, C.c_void_p), None)
+
+    def optimize(self, x0, optimizer="CG-FRPRMN", iterations=50, ftol=1e-12, gtol=1e-12,
+                 lbfgs_k=100):
+        img = np.ascontiguousarray(x0, np.float64).copy()
+        return run_optimizer(self.problem, img, optimizer, iterations, ftol, gtol, lbfgs_k)
 
 
 def set_threads(n: int) -> None:

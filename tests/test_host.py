@@ -1,0 +1,194 @@
+"""Host-side checks that need no GPU: the C ABI libraries load and export every
+declared entry point, host-forcible errors are reported as status codes, the
+set-up and generator agree with the oracle / closed forms, and the factories
+hold the reference's ids."""
+import ctypes as C
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from gpuvmem_amd import native
+from gpuvmem_amd.problem import make_problem, sim_config, small_problem
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared(header: Path) -> set:
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(gvm_[a-z0-9_]+)\s*\(", text))
+
+
+def _exported(lib: Path) -> set:
+    out = subprocess.run(["nm", "-D", "--defined-only", str(lib)], capture_output=True, text=True,
+                         check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_hip_library_exports_every_declared_symbol():
+    decl = _declared(ROOT / "include" / "gvm_hip.h")
+    assert len(decl) >= 35
+    exp = _exported(native.LIB_DIR / "libgvm_hip.so")
+    assert decl <= exp, sorted(decl - exp)
+    # and the bindings know all of them
+    assert decl <= set(native.HIP_API), sorted(decl - set(native.HIP_API))
+
+
+def test_framework_library_exports_every_declared_symbol():
+    exp = _exported(native.LIB_DIR / "libgvm.so")
+    for h in ("fw.h", "setup.h", "simdata.h"):
+        decl = _declared(ROOT / "include" / "gvm" / h)
+        assert decl, h
+        assert decl <= exp, (h, sorted(decl - exp))
+
+
+def test_libraries_load_without_gpu():
+    lib = native.load_hip()
+    native.load_fw()
+    n = C.c_int(-1)
+    st = lib.gvm_device_info(None, 0, C.byref(n), None)
+    assert st in (native.GVM_OK, 5)
+
+
+def test_create_reports_errors_as_status_codes():
+    lib = native.load_hip()
+    p = small_problem()
+    noise = np.ascontiguousarray(p.noise, np.float32)
+    ctx = C.c_void_p()
+    g = p.geometry()
+    g.M = 100  # not square / not a power of two
+    assert lib.gvm_create(C.byref(g), noise.ctypes.data, C.byref(ctx)) == 1
+    assert b"M must equal N" in lib.gvm_last_error()
+    g = p.geometry()
+    g.x0_ref = float(p.M + 3)  # pointing centre outside the image (src/mfs.cu:708-731)
+    assert lib.gvm_create(C.byref(g), noise.ctypes.data, C.byref(ctx)) == 1
+    g = p.geometry()
+    g.image_count = 1
+    assert lib.gvm_create(C.byref(g), noise.ctypes.data, C.byref(ctx)) == 1
+    assert lib.gvm_create(None, None, C.byref(ctx)) == 1
+    # a valid geometry on a machine without a gfx950 device
+    g = p.geometry()
+    st = lib.gvm_create(C.byref(g), noise.ctypes.data, C.byref(ctx))
+    if st == 0:
+        lib.gvm_destroy(ctx)
+    else:
+        assert st == 5
+    # calls on a null context are rejected, not crashed
+    out = C.c_double()
+    assert lib.gvm_chi2(None, None, 0, 0, C.byref(out)) == 1
+    assert lib.gvm_vec_dot(None, None, None, 4, C.byref(out)) == 1
+
+
+def test_primary_beam_table_matches_oracle():
+    lib = native.load_hip()
+    p = small_problem(npix=64)
+    pb = np.zeros((p.N, p.M), np.float32)
+    st = lib.gvm_primary_beam(p.beam_kind, p.antenna_diameter, p.pb_factor, p.pb_cutoff, p.freq,
+                              p.M, p.N, p.x0, p.y0, p.deltax, p.deltay, pb.ctypes.data)
+    assert st == 0
+    ref = oracle.Oracle(p).pb()
+    assert np.allclose(pb, ref, rtol=1e-7, atol=1e-30)
+    assert pb[int(p.y0), int(p.x0)] == 1.0
+    # Airy disk: 1 at the centre, first null at pb_factor lambda / D scaled by RZ
+    pa = np.zeros((p.N, p.M), np.float32)
+    assert lib.gvm_primary_beam(0, 12.0, 1.13, 1.0, p.freq, p.M, p.N, p.x0, p.y0, p.deltax,
+                                p.deltay, pa.ctypes.data) == 0
+    b = oracle.OBeam(0, 12.0, 1.13, 1.0)
+    ref_a = np.array([[oracle.lib().orc_attenuation(C.byref(b), p.freq, i, j, p.x0, p.y0, p.deltax,
+                                                    p.deltay) for j in range(p.M)] for i in range(p.N)])
+    assert np.allclose(pa, ref_a, rtol=1e-6)
+    assert lib.gvm_primary_beam(7, 12.0, 1.13, 1.0, p.freq, p.M, p.N, p.x0, p.y0, p.deltax,
+                                p.deltay, pa.ctypes.data) == 1
+
+
+def test_setup_matches_oracle_restatement():
+    fw = native.load_fw()
+    cfg = sim_config(0, n_antennas=6, dump_seconds=120.0)
+    n = fw.gvm_sim_count_visibilities(C.byref(cfg))
+    uvw = np.zeros((n, 3))
+    vo = np.zeros((n, 2), np.float32)
+    w = np.zeros(n, np.float32)
+    assert fw.gvm_sim_generate(C.byref(cfg), uvw.ctypes.data, vo.ctypes.data, w.ctypes.data, None,
+                               None, None) == 0
+    freq = np.float32(cfg.freq_hz)
+    out = [C.c_double() for _ in range(6)]
+    fq = (C.c_float * 1)(freq)
+    nv = (C.c_long * 1)(n)
+    up = (C.c_void_p * 1)(uvw.ctypes.data)
+    wp = (C.c_void_p * 1)(w.ctypes.data)
+    assert fw.gvm_setup_beam_noise(1, fq, nv, up, wp, -0.05 / 3600, *[C.byref(o) for o in out]) == 0
+    ch = (oracle.OChan * 1)(oracle.OChan(freq, n, uvw.ctypes.data, vo.ctypes.data, w.ctypes.data))
+    ref = [C.c_double() for _ in range(5)]
+    oracle.lib().orc_beam_and_noise(ch, 1, -0.05 / 3600, *[C.addressof(r) for r in ref])
+    for a, b in zip(out[:5], ref):
+        assert np.isclose(a.value, b.value, rtol=1e-12)
+    njp = oracle.lib().orc_noise_jypix(ref[4].value, ref[0].value, ref[1].value, -0.05 / 3600)
+    assert np.isclose(out[5].value, njp, rtol=1e-12)
+    # zero weights are rejected ("The sum of the visibility weights cannot be zero")
+    w0 = np.zeros(n, np.float32)
+    wp0 = (C.c_void_p * 1)(w0.ctypes.data)
+    assert fw.gvm_setup_beam_noise(1, fq, nv, up, wp0, -0.05 / 3600, *[C.byref(o) for o in out]) == 1
+    # Hermitian flip and wavelengths
+    a, va = uvw.copy(), vo.copy()
+    b, vb = uvw.copy(), vo.copy()
+    fw.gvm_setup_hermitian_lambda(n, a.ctypes.data, va.ctypes.data, freq)
+    oracle.lib().orc_hermitian_to_lambda(n, b.ctypes.data, vb.ctypes.data, freq)
+    assert np.array_equal(a, b) and np.array_equal(va, vb)
+    assert np.all(a[:, 0] <= 0)
+
+
+def test_generator_counts_and_determinism():
+    fw = native.load_fw()
+    for which, n in ((0, 36000), (1, 1764000)):
+        cfg = sim_config(which)
+        assert fw.gvm_sim_count_visibilities(C.byref(cfg)) == n
+    p1 = small_problem(seed=5)
+    p2 = small_problem(seed=5)
+    assert np.array_equal(p1.uvw, p2.uvw) and np.array_equal(p1.vo, p2.vo)
+    p3 = small_problem(seed=5, seed_noise=99)
+    assert np.array_equal(p1.uvw, p3.uvw) and not np.array_equal(p1.vo, p3.vo)
+    assert np.all(p1.weight == np.float32(1e6))
+    # noise statistics: sigma 1 mJy per component
+    d = (p1.vo - p3.vo).ravel()
+    assert 0.8e-3 < d.std() / np.sqrt(2) < 1.2e-3
+
+
+def test_problem_setup_invariants():
+    p = make_problem(0)
+    assert p.nvis == 36000 and p.M == p.N == 256
+    assert np.all(p.uvw[:, 0] <= 0)
+    assert np.isclose(p.fg_scale, p.beam_info["noise_jypix"], rtol=1e-6)
+    assert np.isclose(p.noise_cut, 10 * p.fg_scale, rtol=1e-6)
+    assert p.noise[int(p.y0), int(p.x0)] == np.float32(p.fg_scale)
+
+
+def test_factories_hold_reference_ids():
+    fw = native.load_fw()
+    for name in ("Chi2", "Entropy", "TotalSquaredVariation", "Laplacian"):
+        assert fw.gvm_fw_registered(b"Fi", name.encode()) == 1, name
+    for name in ("CG-FRPRMN", "CG-LBFGS"):
+        assert fw.gvm_fw_registered(b"Optimizer", name.encode()) == 1, name
+    assert fw.gvm_fw_registered(b"ObjectiveFunction", b"ObjectiveFunction") == 1
+    assert fw.gvm_fw_registered(b"Fi", b"L1-Norm") == 0  # not on this path
+    assert fw.gvm_fw_registered(b"Optimizer", b"nope") == 0
+
+
+def test_bench_refuses_several_gpus():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_oracle_is_not_reachable_from_the_product():
+    """The product package never imports, links or opens the oracle."""
+    for f in list((ROOT / "gpuvmem_amd").rglob("*.py")) + list((ROOT / "gpuvmem_amd").rglob("*.cpp")) \
+            + list((ROOT / "gpuvmem_amd").rglob("*.hip")):
+        t = f.read_text()
+        assert not re.search(r"^\s*(import|from)\s+oracle|liboracle|gvm_oracle", t, re.M), f
+    ldd = subprocess.run(["ldd", str(native.LIB_DIR / "libgvm.so")], capture_output=True, text=True).stdout
+    assert "oracle" not in ldd
