@@ -144,6 +144,8 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned int* hi,
 }
 
 constexpr int kGemmThreads = 512;
+constexpr int kPScale = 15;  // max |P| = 2^15 < 65504 (FP16 max)
+constexpr int kQScale = 14;
 constexpr int kHalfKV = kKV / 2;
 
 // One workgroup (8 waves, 2 per SIMD) owns a 256x256 tile of G for one
@@ -193,8 +195,12 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
   const float2 xt = cvalid ? xtab[col] : make_float2(0.f, 0.f);
 
   const float cm = __uint_as_float(*cmax);
-  const float scale = cm > 0.0f ? exp2f(-ceilf(log2f(cm))) : 1.0f;
+  // Operand scaling (powers of two, undone in the epilogue): |P| <= 2^kPScale
+  // and |Q| <= 2^kQScale keep the lo parts of the FP16 split in the normal
+  // range instead of the subnormals.
+  const float scale = cm > 0.0f ? exp2f((float)kPScale - ceilf(log2f(cm))) : 1.0f;
   const float rscale = rvalid ? scale : 0.0f;  // P of rows past the image is 0
+  constexpr float qscale = (float)(1 << kQScale);
 
   const int lr = lane & 31, lh = lane >> 5;
   const int wr = (wave & 1) * 128, wc = (wave >> 1) * 64;
@@ -227,8 +233,8 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
       const float amp = d.x * rscale;
       pv[2 * q] = amp * cy;
       pv[2 * q + 1] = -(amp * sy);
-      qv[2 * q] = cx;
-      qv[2 * q + 1] = sx;
+      qv[2 * q] = cx * qscale;
+      qv[2 * q + 1] = sx * qscale;
     }
     if constexpr (PREC == PREC_F32) {
 #pragma unroll
@@ -422,7 +428,8 @@ __global__ void adjoint_epilogue_kernel(const float* __restrict__ part, int nspl
   double inv_scale = 1.0;
   if (scaled) {
     const float cm = __uint_as_float(*cmax);
-    inv_scale = cm > 0.0f ? (double)exp2f(ceilf(log2f(cm))) : 1.0;
+    inv_scale = (cm > 0.0f ? (double)exp2f(ceilf(log2f(cm)) - (float)kPScale) : 1.0) *
+                exp2(-(double)kQScale);
   }
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
        k += (long)gridDim.x * blockDim.x) {
@@ -443,14 +450,21 @@ __global__ void adjoint_epilogue_kernel(const float* __restrict__ part, int nspl
 
 namespace {
 
-int choose_nsplit(long nvis, int ntiles, int cu_count) {
-  long target_wg = 4L * std::max(cu_count, 64);
-  long ns = (target_wg + ntiles - 1) / ntiles;
-  ns = std::max(ns, (nvis + 32767) / 32768);  // bound the FP32 accumulation length
-  ns = std::min(ns, std::max(1L, nvis / 512));
+int choose_nsplit(long nvis, int ntiles, int cu_count, long n_pixels) {
+  // Each split accumulates its visibilities in FP32 MFMA accumulators; the
+  // splits are combined in FP64.  The FP32 rounding error grows with the
+  // square root of the accumulation length, so splits are kept near kSplitVis
+  // visibilities (about 10^3 accumulation steps), within a workspace of
+  // kMaxWorkspace bytes, and at least enough splits to fill the GPU.
+  constexpr long kSplitVis = 3072;
+  constexpr long kMaxWorkspace = 4L << 30;
+  long ns = (nvis + kSplitVis - 1) / kSplitVis;
+  const long fill = (4L * std::max(cu_count, 64) + ntiles - 1) / ntiles;
+  ns = std::max(ns, std::min(fill, std::max(1L, nvis / 512)));
+  ns = std::min(ns, std::max(8L, kMaxWorkspace / (long)(sizeof(float) * n_pixels)));
   ns = std::max(ns, 1L);
   ns = ((ns + 7) / 8) * 8;  // the XCD remap needs multiples of 8 workgroups
-  return (int)std::min(ns, 1024L);
+  return (int)std::min(ns, 4096L);
 }
 
 }  // namespace
@@ -465,7 +479,7 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
   for (size_t si = 0; si < ctx->slices.size(); si++) {
     const Slice& s = ctx->slices[si];
     const int mode = ctx->adjoint_mode;
-    const int nsplit = mode == GVM_ADJOINT_DIRECT ? 1 : choose_nsplit(s.nvis, ntiles, ctx->cu_count);
+    const int nsplit = mode == GVM_ADJOINT_DIRECT ? 1 : choose_nsplit(s.nvis, ntiles, ctx->cu_count, n);
     const size_t need = (size_t)nsplit * n;
     if (need > ctx->adj_part_elems) {
       GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -168,7 +168,9 @@ class Oracle:
     ``noise`` defaults to the problem's float32 noise image so the mask is
     bit-identical to the one the device uses."""
 
-    def __init__(self, problem, separable: bool = True, noise=None):
+    def __init__(self, problem, separable: bool = True, noise=None, slices=None):
+        """slices: optional list of (freq, uvw, vo, weight) replacing the
+        problem's single channel."""
         self.p = problem
         L = lib()
         self.noise = np.ascontiguousarray(problem.noise if noise is None else noise, np.float32)
@@ -181,11 +183,20 @@ class Oracle:
                           problem.threshold, problem.nu_0, _ptr(self.noise))
         self.beam = OBeam(problem.beam_kind, problem.antenna_diameter, problem.pb_factor,
                           problem.pb_cutoff)
-        self.uvw = np.ascontiguousarray(problem.uvw, np.float64)
-        self.vo = np.ascontiguousarray(problem.vo, np.float32)
-        self.weight = np.ascontiguousarray(problem.weight, np.float32).copy()
-        self.chan = (OChan * 1)(OChan(problem.freq, self.uvw.shape[0], _ptr(self.uvw),
-                                      _ptr(self.vo), _ptr(self.weight)))
+        if slices is None:
+            slices = [(problem.freq, problem.uvw, problem.vo, problem.weight)]
+        self._keep = []
+        chans = []
+        for f, u, v, w in slices:
+            u = np.ascontiguousarray(u, np.float64)
+            v = np.ascontiguousarray(v, np.float32)
+            w = np.ascontiguousarray(w, np.float32).copy()
+            self._keep.append((u, v, w))
+            chans.append(OChan(f, u.shape[0], _ptr(u), _ptr(v), _ptr(w)))
+        self.nch = len(chans)
+        self.chan = (OChan * self.nch)(*chans)
+        self.uvw, self.vo, self.weight = self._keep[0]
+        self.ntot = sum(k[0].shape[0] for k in self._keep)
         self.separable = separable
         self.n = problem.M * problem.N
         self._lib = L
@@ -194,9 +205,9 @@ class Oracle:
     def chi2(self, image, flag_opt: int = 0, normalize: bool = False):
         """Returns (chi2, residuals complex128, clipped image float64)."""
         img = np.ascontiguousarray(image, np.float64).reshape(2, self.p.N, self.p.M).copy()
-        vr = np.zeros(self.uvw.shape[0], np.complex128)
-        v = self._lib.orc_chi2(C.byref(self.geom), C.byref(self.beam), self.chan, 1, _ptr(img),
-                               flag_opt, int(normalize), _ptr(vr))
+        vr = np.zeros(self.ntot, np.complex128)
+        v = self._lib.orc_chi2(C.byref(self.geom), C.byref(self.beam), self.chan, self.nch,
+                               _ptr(img), flag_opt, int(normalize), _ptr(vr))
         return v, vr, img
 
     def model_grid(self, image):
@@ -219,8 +230,8 @@ class Oracle:
         vr = np.ascontiguousarray(vr, np.complex128)
         res = np.zeros((2, self.p.N, self.p.M), np.float64)
         sep = self.separable if separable is None else separable
-        self._lib.orc_dchi2(C.byref(self.geom), C.byref(self.beam), self.chan, 1, _ptr(img),
-                            _ptr(vr), _ptr(res), flag_opt, int(normalize), int(sep))
+        self._lib.orc_dchi2(C.byref(self.geom), C.byref(self.beam), self.chan, self.nch,
+                            _ptr(img), _ptr(vr), _ptr(res), flag_opt, int(normalize), int(sep))
         return res
 
     def dchi2_channel(self, vr, normalize: bool = False, separable=None):
@@ -272,7 +283,7 @@ class Oracle:
         of.g = C.pointer(self.geom)
         of.b = C.pointer(self.beam)
         of.ch = C.cast(self.chan, C.POINTER(OChan))
-        of.nch = 1
+        of.nch = self.nch
         of.image_count = 2
         of.flag_opt = flag_opt
         of.normalize = 0
