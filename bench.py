@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -81,6 +82,37 @@ def cpu_baseline(p, threads: int, target_s: float) -> dict:
     }
 
 
+def profiled_traffic(mode: str, nvis: int):
+    """HBM bytes per launch of the adjoint GEMM from the committed rocprofv3
+    counter summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
+    known size), when one matches this adjoint form and visibility count."""
+    pat = re.compile(r"^traffic_bytes_per_launch mode=(\S+) nvis=(\d+) bytes=(\d+)", re.M)
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for m in pat.finditer(f.read_text()):
+            if m.group(1) == mode and int(m.group(2)) == nvis:
+                return int(m.group(3)), f.name
+    return None, None
+
+
+def copy_calibration(ctx, mib: int) -> None:
+    """A plain device-to-device copy of known size (the library's copy
+    kernel), outside the timed region, so counter runs can scale FETCH_SIZE /
+    WRITE_SIZE against it."""
+    from gpuvmem_amd import native
+    from gpuvmem_amd.device import DeviceBuffer
+
+    n = (mib << 20) // 4
+    src, dst = DeviceBuffer(4 * n), DeviceBuffer(4 * n)
+    native.check(ctx.lib.gvm_vec_zero(ctx.ctx, src.ptr, n), "zero")
+    for _ in range(3):
+        native.check(ctx.lib.gvm_vec_copy(ctx.ctx, dst.ptr, src.ptr, n), "copy")
+    ctx.sync()
+    src.free()
+    dst.free()
+    print(f"# copy calibration: copy_kernel of {4 * n} bytes read and {4 * n} bytes written, "
+          "3 launches", file=sys.stderr, flush=True)
+
+
 def _slice_problem(p, k):
     import copy
 
@@ -100,6 +132,8 @@ def main() -> int:
     ap.add_argument("--workload", type=int, default=1, help="1: 1024^2/1.76M vis, 0: 256^2/36k")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this size (for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -136,6 +170,7 @@ def main() -> int:
     print(f"# {args.steps} steps in {elapsed:.3f} s; adjoint kernel {adj_ms:.3f} ms avg over "
           f"{adj_n} launches", file=sys.stderr, flush=True)
 
+    traffic, traffic_src = profiled_traffic(args.adjoint, p.nvis)
     flops = 4.0 * p.M * p.N * p.nvis
     achieved = flops / (adj_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.adjoint]
@@ -165,7 +200,9 @@ def main() -> int:
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes/launch (HBM, rocprofv3 FETCH_SIZE+WRITE_SIZE)",
+            "traffic_source": traffic_src,
             "kernel": f"adjoint_gemm_kernel ({args.adjoint})",
             "kernel_ms": adj_ms,
             "algorithmic_flop_per_launch": flops,
@@ -175,6 +212,8 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(p, threads, args.cpu_seconds)
     print(json.dumps(line), flush=True)
+    if args.calibrate_copy_mib > 0:
+        copy_calibration(ctx, args.calibrate_copy_mib)
     obj.close()
     ctx.close()
     return 0

@@ -434,8 +434,18 @@ __global__ void adjoint_epilogue_kernel(const float* __restrict__ part, int nspl
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
        k += (long)gridDim.x * blockDim.x) {
     if (!(noise[k] < noise_cut)) continue;
+    // FP64 sum over the splits in split order; eight loads are issued before
+    // their adds so each thread keeps several HBM requests in flight.
     double g = 0.0;
-    for (int s = 0; s < nsplit; s++) g += (double)part[(long)s * n + k];
+    int s = 0;
+    for (; s + 8 <= nsplit; s += 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = part[(long)(s + u) * n + k];
+#pragma unroll
+      for (int u = 0; u < 8; u++) g += (double)v[u];
+    }
+    for (; s < nsplit; s++) g += (double)part[(long)s * n + k];
     // DChi2: dchi2 = -(sum * fg_scale * atten) [/ nvis]
     const float d = (float)(-(g * inv_scale) * (double)(fg_scale * pb[k]) * inv_nvis);
     const float i0 = I[k], alpha = I[n + k];
@@ -515,7 +525,8 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
     GVM_CHECK_LAUNCH();
     GVM_HIP_TRY(hipEventRecord(ctx->ev_b, ctx->stream));
     const double inv_nvis = normalize ? 1.0 / (double)s.nvis : 1.0;
-    hipLaunchKernelGGL(adjoint_epilogue_kernel, dim3(reduce_grid(n)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(adjoint_epilogue_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream,
                        ctx->adj_part, nsplit, n, ctx->cmax + si,
                        mode == GVM_ADJOINT_DIRECT ? 0 : 1, ctx->noise, g.noise_cut, s.pb,
                        g.fg_scale, inv_nvis, s.freq / g.nu_0, g.threshold, flag_opt, I, result);

@@ -150,12 +150,20 @@ def test_full_size_chi2_and_gradient_subset(full):
 
 def test_full_size_precision_forms_agree(full):
     """Size-independent property: the FP16 and BF16 split forms agree with the
-    FP32 MFMA form over the whole image (each within its error budget)."""
+    FP32 MFMA form over the whole image, and each form agrees with FP64 direct
+    sums on a pixel sample."""
     p = full
     img = p.initial_image()
     img[0] = np.maximum(p.sky / p.fg_scale * 0.5, p.minpix).astype(np.float32)
-    out = {}
+    orc = oracle.Oracle(p)
+    _, vr, _ = orc.chi2(img)
+    rng = np.random.default_rng(1)
+    inside = np.argwhere(p.noise < p.noise_cut)
+    pix = inside[rng.choice(len(inside), 32, replace=False)]
+    gref = orc.dchi2_pixels(vr, pix)
+    out, sub = {}, {}
     for mode in ("f32", "f16x3", "bf16x3"):
+        print(f"case: full size adjoint {mode}", flush=True)
         ctx = Context(p, adjoint=mode)
         d = ctx.image(img)
         ctx.chi2(d)
@@ -163,9 +171,13 @@ def test_full_size_precision_forms_agree(full):
         ctx.dchi2(d, res)
         ctx.sync()
         out[mode] = res.download(np.float32, (2, p.N, p.M))[0].astype(np.float64)
+        gs = np.array([out[mode][i, j] for i, j in pix])
+        sub[mode] = np.linalg.norm(gs - gref) / np.linalg.norm(gref)
         ctx.close()
     ref = out["f32"]
     e16 = np.linalg.norm(out["f16x3"] - ref) / np.linalg.norm(ref)
     eb16 = np.linalg.norm(out["bf16x3"] - ref) / np.linalg.norm(ref)
-    print(f"f16x3 vs f32 {e16:.3e}; bf16x3 vs f32 {eb16:.3e}")
-    assert e16 < 2e-6 and eb16 < 1e-4
+    print(f"f16x3 vs f32 {e16:.3e}; bf16x3 vs f32 {eb16:.3e}; vs FP64 on pixels: "
+          + ", ".join(f"{m} {e:.3e}" for m, e in sub.items()))
+    assert sub["f32"] < GRAD_TOL and sub["f16x3"] < GRAD_TOL
+    assert e16 < GRAD_TOL and eb16 < 1e-4
