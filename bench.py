@@ -71,6 +71,7 @@ def cpu_baseline(p, threads: int, target_s: float) -> dict:
         frac_n = min(n, int(frac_n * max(2.0, (target_s / 4) / max(dt, 1e-3))))
     t_grad = dt * n / frac_n
     evals = 1.0 / (t_fwd + t_grad)
+    grid_s, grid_k = gridding_comparator(p, oracle, threads)
     return {
         "value": evals,
         "unit": "evals/s",
@@ -79,7 +80,35 @@ def cpu_baseline(p, threads: int, target_s: float) -> dict:
         "sample": (f"FP64 oracle: full-size forward chi2 ({t_fwd:.2f} s) + separable FP64 "
                    f"gradient over {frac_n} of {n} visibilities ({dt:.2f} s), gradient "
                    f"extrapolated x{n / frac_n:.1f} to {t_grad:.1f} s"),
+        "gridding_comparator": {
+            "seconds_per_grid": grid_s,
+            "threads": threads,
+            "sample": (f"CPU restatement of the OpenMP gridding path do_gridding "
+                       f"(src/functions.cu:1339-1653), 7x7 pillbox, {grid_k} of {n} visibilities "
+                       f"(both Hermitian halves), extrapolated linearly"),
+        },
     }
+
+
+def gridding_comparator(p, oracle, threads: int, k: int = 100000):
+    """Times the CPU gridding comparator on k visibilities, returns the time
+    extrapolated to the whole observation and k."""
+    import ctypes as C
+
+    k = min(k, p.nvis)
+    lam = float(np.float32(2.99792458e8) / np.float32(p.freq))
+    uvw = np.ascontiguousarray(p.uvw[:k] * lam)  # metres, as the gridder reads them
+    vo = np.ascontiguousarray(p.vo[:k])
+    w = np.ascontiguousarray(p.weight[:k])
+    ch = (oracle.OChan * 1)(oracle.OChan(p.freq, k, uvw.ctypes.data, vo.ctypes.data, w.ctypes.data))
+    n = p.M * p.N
+    re, im, gw = np.zeros(n), np.zeros(n), np.zeros(n)
+    du = 1.0 / (p.M * p.deltax * np.pi / 180.0)
+    dv = 1.0 / (p.N * p.deltay * np.pi / 180.0)
+    t0 = time.perf_counter()
+    oracle.lib().orc_do_gridding_pillbox(p.M, p.N, du, dv, ch, 1, 3, threads, re.ctypes.data,
+                                         im.ctypes.data, gw.ctypes.data)
+    return (time.perf_counter() - t0) * p.nvis / k, k
 
 
 def profiled_traffic(mode: str, nvis: int):

@@ -157,6 +157,15 @@ extern "C" gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimiz
   return GVM_OK;
 }
 
+extern "C" gvm_status gvm_fw_set_flag(gvm_fw_objective* obj, int flag_opt) {
+  if (!obj || flag_opt < 0) {
+    g_fw_error = "gvm_fw_set_flag: bad argument";
+    return GVM_ERR_ARG;
+  }
+  obj->cfg.flag_opt = flag_opt;
+  return GVM_OK;
+}
+
 extern "C" int gvm_fw_registered(const char* kind, const char* name) {
   if (!kind || !name) return 0;
   const std::string k(kind), n(name);

@@ -216,6 +216,19 @@ class Objective:
         n = min(res.iterations, iterations)
         return res, [traj[k] for k in range(n)]
 
+    def set_flag(self, flag_opt: int) -> None:
+        """Optimizer::setFlag for the following calls (schedule flag)."""
+        native.check_fw(self.fw.gvm_fw_set_flag(self.h, flag_opt), "gvm_fw_set_flag")
+
+    def optimize_schedule(self, img: DeviceBuffer, flags=(0, 1, 2, 3), **kw):
+        """MFS::run's schedule (src/mfs.cu:995-1009): the optimiser runs once
+        per flag, in order, on the same image.  Returns [(res, traj)]."""
+        out = []
+        for f in flags:
+            self.set_flag(f)
+            out.append(self.optimize(img, **kw))
+        return out
+
     def close(self) -> None:
         if self.h:
             self.fw.gvm_fw_objective_destroy(self.h)

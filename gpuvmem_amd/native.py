@@ -146,6 +146,7 @@ FW_API = {
     "gvm_fw_gradient": ([_P, _P, _P, _I], _I),
     "gvm_fw_term_values": ([_P, _DP, _I, _IP], _I),
     "gvm_fw_optimize": ([_P, C.c_char_p, _I, _D, _D, _I, _P, C.POINTER(FwResult), _DP, _I], _I),
+    "gvm_fw_set_flag": ([_P, _I], _I),
     "gvm_fw_last_error": ([], C.c_char_p),
     "gvm_fw_registered": ([C.c_char_p, C.c_char_p], _I),
 }

@@ -66,6 +66,12 @@ gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimizer, int tot
                            double ftol, double gtol, int lbfgs_k, float* p_dev,
                            gvm_fw_result* res, double* traj, int cap);
 
+/* Optimizer::setFlag for the next gvm_fw_optimize / value / gradient: the
+ * schedule flag_opt (image flag_opt % 2 is optimised; flag_opt > 0 zeroes
+ * alpha where I_nu_0 < threshold).  MFS::run (src/mfs.cu:995-1009) runs the
+ * optimiser with flags 0, 1, 2, 3 in turn on the same image. */
+gvm_status gvm_fw_set_flag(gvm_fw_objective* obj, int flag_opt);
+
 const char* gvm_fw_last_error(void);
 
 /* 1 if `name` is registered with the factory of `kind` ("Fi", "Optimizer",

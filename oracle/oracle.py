@@ -327,6 +327,19 @@ class Oracle:
         return run_optimizer(pr, img, optimizer, iterations, ftol, gtol, lbfgs_k)
 
 
+    def optimize_schedule(self, of, image, flags=(0, 1, 2, 3), **kw):
+        """MFS::run's schedule (src/mfs.cu:995-1009): one optimiser run per
+        schedule flag, in order, on the same image.  Returns (final image,
+        [(OResult, trajectory)])."""
+        img = np.ascontiguousarray(image, np.float64).copy()
+        out = []
+        for f in flags:
+            of.flag_opt = f
+            img, res, traj = self.optimize(of, img, **kw)
+            out.append((res, traj))
+        return img, out
+
+
 def run_optimizer(pr: OProblem, img: np.ndarray, optimizer: str, iterations: int,
                   ftol: float = 1e-12, gtol: float = 1e-12, lbfgs_k: int = 100):
     """Runs frprmn ("CG-FRPRMN") or lbfgs ("CG-LBFGS") on img in place."""

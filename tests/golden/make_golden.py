@@ -17,7 +17,7 @@ on library and brute-force arithmetic, not on the C oracle:
 * the Gaussian primary beam (src/functions.cu:2293-2333) and clip2IWNoise /
   calculateInu (src/functions.cu:2694-2719, 3939-3966) by their formulas.
 
-Run ``python tests/golden/make_golden.py`` to regenerate ``forward_adjoint_32.npz``.
+Run ``python tests/golden/make_golden.py`` to regenerate ``forward_adjoint_64.npz``.
 """
 from __future__ import annotations
 
@@ -30,9 +30,9 @@ LIGHTSPEED = 2.99792458e8
 
 
 def geometry():
-    npix = 32
+    npix = 64
     return dict(
-        M=npix, N=npix, deltax=-0.05 / 3600.0, deltay=0.05 / 3600.0, x0=16.0, y0=16.0,
+        M=npix, N=npix, deltax=-0.05 / 3600.0, deltay=0.05 / 3600.0, x0=32.0, y0=32.0,
         freq=float(np.float32(230e9)), nu_0=float(np.float32(230e9 / 1.05)),
         beam_kind=1, antenna_diameter=12.0, pb_factor=1.13, pb_cutoff=10.0,
         minpix=0.001, eta=-1.0, threshold=0.0, fg_scale=0.375, noise_cut=2.0,
@@ -133,10 +133,10 @@ def make():
     nvis = 300
     du = 1.0 / (g["M"] * abs(g["deltax"]) * np.pi / 180.0)
     uvw = np.zeros((nvis, 3))
-    uvw[:, 0] = -rng.uniform(0.0, 15.0, nvis) * du          # Hermitian half: u <= 0
-    uvw[:, 1] = rng.uniform(-15.0, 15.0, nvis) * du
+    uvw[:, 0] = -rng.uniform(0.0, 31.0, nvis) * du          # Hermitian half: u <= 0
+    uvw[:, 1] = rng.uniform(-31.0, 31.0, nvis) * du
     uvw[:, 2] = rng.uniform(-3e4, 3e4, nvis)
-    uvw[::37, 0] = -40.3 * du                                 # column 40.3 >= N: off the grid
+    uvw[::37, 0] = -70.3 * du                                 # column 70.3 >= N: off the grid
     vo = (rng.standard_normal((nvis, 2)) * 0.05).astype(np.float32)
     weight = rng.uniform(0.5, 2.0, nvis).astype(np.float32)
     noise = np.full((n, n), 1.0, np.float32)
@@ -156,7 +156,7 @@ def make():
     g0 = dchi2_direct(vr, uvw, w, clipped, noise, pb, g, 0)
     g1 = dchi2_direct(vr, uvw, w, clipped, noise, pb, g, 1)
     np.savez_compressed(
-        HERE / "forward_adjoint_32.npz",
+        HERE / "forward_adjoint_64.npz",
         geometry=np.array([repr(g)]), uvw=uvw, vo=vo, weight=weight, noise=noise, image=img,
         pb=pb, clipped=clipped, vr=vr, chi2=np.array(chi2), weight_after=w,
         grad_flag0=g0, grad_flag1=g1,

@@ -15,7 +15,7 @@ GOLD = Path(__file__).resolve().parent / "golden"
 
 
 def _load():
-    z = np.load(GOLD / "forward_adjoint_32.npz")
+    z = np.load(GOLD / "forward_adjoint_64.npz")
     g = ast.literal_eval(str(z["geometry"][0]))
     p = Problem(M=g["M"], N=g["N"], deltax=g["deltax"], deltay=g["deltay"], x0=g["x0"], y0=g["y0"],
                 freq=g["freq"], nu_0=g["nu_0"], uvw=z["uvw"], vo=z["vo"], weight=z["weight"],
@@ -56,7 +56,7 @@ def test_hip_path_matches_golden():
     from gpuvmem_amd.device import Context
 
     p, z = _load()
-    print(f"case: golden 32^2, nvis={p.nvis}", flush=True)
+    print(f"case: golden 64^2, nvis={p.nvis}", flush=True)
     ref_chi2 = float(z["chi2"])
     for mode in ("f16x3", "f32", "bf16x3", "direct"):
         ctx = Context(p, adjoint=mode)

@@ -137,6 +137,55 @@ def test_lbfgs_on_device_is_memory_one_for_image0():
     assert abs(r10.final_value - ores.final_value) <= 0.2 * abs(ores.final_value)
 
 
+def test_mfs_flag_schedule_two_channels():
+    """MFS::run's schedule on a two-channel problem with a spectral index:
+    CG-FRPRMN with flags 0 (I_nu_0), 1 (alpha), 2, 3 in turn on the same
+    image, against the oracle running the same schedule.  Each stage is
+    compared on its first iteration and its final value; alpha moves only in
+    the odd stages, I_nu_0 only in the even ones."""
+    from gpuvmem_amd.device import DeviceBuffer
+
+    p = small_problem(npix=64, n_antennas=7, seed=41)
+    q = small_problem(npix=64, n_antennas=7, seed=41, freq_hz=250e9, seed_noise=77)
+    slices = [(p.freq, p.uvw, p.vo, p.weight), (q.freq, q.uvw, q.vo, q.weight)]
+    p.threshold = 0.01
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001)]
+    iters = 4
+    print(f"case: MFS schedule, 2 channels, {p.M}^2, nvis={p.nvis}+{q.nvis}", flush=True)
+    ctx = Context(p, add_data=False)
+    for f, u, v, w in slices:
+        ctx.add(f, u, v, w)
+    obj = Objective(ctx, fw_config(terms, lambdas=(0.01,)))
+    d = ctx.image(p.initial_image())
+    images = [p.initial_image()]
+    stages = []
+    for flag in (0, 1, 2, 3):
+        obj.set_flag(flag)
+        stages.append(obj.optimize(d, "CG-FRPRMN", iters))
+        images.append(d.download(np.float32, (2, p.N, p.M)))
+    obj.close()
+    ctx.close()
+    orc = oracle.Oracle(p, slices=slices)
+    of = orc.objective(terms, lambdas=(0.01,))
+    ref, ostages = orc.optimize_schedule(of, p.initial_image(), optimizer="CG-FRPRMN",
+                                         iterations=iters)
+    orc.free(of)
+    for k, ((res, traj), (ores, otraj)) in enumerate(zip(stages, ostages)):
+        print(f"stage flag {k}: gpu f {res.final_value:.8e} ({res.iterations} it), oracle "
+              f"{ores.final_value:.8e} ({ores.iterations} it); first {traj[0]:.8e} vs {otraj[0]:.8e}")
+        # (not monotone: at flag > 0 the clip zeroes alpha where I_nu_0 < threshold,
+        # in place, so the oracle's own values can rise within a stage)
+        assert abs(traj[0] - otraj[0]) <= 1e-3 * abs(otraj[0]), k
+        assert abs(res.final_value - ores.final_value) <= 0.05 * abs(ores.final_value), k
+    for k in range(4):
+        moved, fixed = k % 2, 1 - k % 2
+        assert np.array_equal(images[k + 1][fixed], images[k][fixed]), k
+        assert not np.array_equal(images[k + 1][moved], images[k][moved]), k
+    ierr = np.linalg.norm(images[-1] - ref) / np.linalg.norm(ref)
+    print(f"final image rel L2 {ierr:.3e}")
+    assert ierr < 0.1
+
+
 def test_unknown_optimizer_is_an_error():
     from gpuvmem_amd import native
 
