@@ -57,18 +57,19 @@ def cpu_baseline(p, threads: int, target_s: float) -> dict:
     t0 = time.perf_counter()
     _, vr, clipped = orc.chi2(img)
     t_fwd = time.perf_counter() - t0
-    # gradient on a slice of the visibilities; grow the slice until it costs
-    # about target_s seconds, then extrapolate linearly in Nvis
-    n = p.nvis
-    frac_n = max(256, n // 512)
+    # gradient on a slice of the visibilities (of the first channel); grow the
+    # slice until it costs about target_s seconds, then extrapolate linearly
+    # in the total visibility count
+    n = p.nvis_total
+    frac_n = max(256, p.nvis // 512)
     while True:
         sub = oracle.Oracle(_slice_problem(p, frac_n), separable=True)
         t0 = time.perf_counter()
         sub.dchi2_channel(vr[:frac_n])
         dt = time.perf_counter() - t0
-        if dt > target_s / 4 or frac_n >= n:
+        if dt > target_s / 4 or frac_n >= p.nvis:
             break
-        frac_n = min(n, int(frac_n * max(2.0, (target_s / 4) / max(dt, 1e-3))))
+        frac_n = min(p.nvis, int(frac_n * max(2.0, (target_s / 4) / max(dt, 1e-3))))
     t_grad = dt * n / frac_n
     evals = 1.0 / (t_fwd + t_grad)
     grid_s, grid_k = gridding_comparator(p, oracle, threads)
@@ -108,7 +109,7 @@ def gridding_comparator(p, oracle, threads: int, k: int = 100000):
     t0 = time.perf_counter()
     oracle.lib().orc_do_gridding_pillbox(p.M, p.N, du, dv, ch, 1, 3, threads, re.ctypes.data,
                                          im.ctypes.data, gw.ctypes.data)
-    return (time.perf_counter() - t0) * p.nvis / k, k
+    return (time.perf_counter() - t0) * p.nvis_total / k, k
 
 
 def profiled_traffic(mode: str, nvis: int):
@@ -146,10 +147,58 @@ def _slice_problem(p, k):
     import copy
 
     q = copy.copy(p)
+    q.slices = None
     q.uvw = p.uvw[:k]
     q.vo = p.vo[:k]
     q.weight = p.weight[:k]
     return q
+
+
+def time_to_image(p, iterations: int, forms=("f32", "f16x3")) -> dict:
+    """L-BFGS (memory 10) from the default initial image with each adjoint
+    form.  The plain FP32 form runs `iterations` iterations; its final
+    objective value is the target, and every form is timed to the first
+    iteration whose value reaches it (up to twice as many iterations).  Each
+    context does one untimed evaluation first (workspace allocation)."""
+    from gpuvmem_amd.device import Context, Objective, fw_config
+
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
+    out, target, images = {}, None, {}
+    for mode in forms:
+        ctx = Context(p, adjoint=mode)
+        obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
+        warm = ctx.image(p.initial_image())
+        xi = ctx.zeros(2 * p.M * p.N)
+        obj.value(warm)
+        obj.gradient(warm, xi, 0)
+        ctx.sync()
+        d = ctx.image(p.initial_image())
+        n = iterations if target is None else 2 * iterations
+        res, traj, secs = obj.optimize_timed(d, "CG-LBFGS", n, lbfgs_k=10)
+        if target is None:
+            target = traj[-1]
+        hit = next((k for k, v in enumerate(traj) if v <= target), None)
+        obj.value(d)
+        chi2 = obj.term_values()[0]
+        images[mode] = d.download(np.float32, (2, p.N, p.M))
+        out[mode] = {
+            "seconds_to_target": secs[hit] if hit is not None else None,
+            "iterations_to_target": hit + 1 if hit is not None else None,
+            "iterations_run": res.iterations,
+            "seconds_run": res.seconds,
+            "final_objective": res.final_value,
+            "final_chi2": chi2,
+            "function_calls": res.function_calls,
+            "gradient_calls": res.gradient_calls,
+        }
+        obj.close()
+        ctx.close()
+        print(f"# time-to-image {mode}: {out[mode]}", file=sys.stderr, flush=True)
+    ref = images[forms[0]]
+    for mode in forms[1:]:
+        out[mode]["image_rel_l2_vs_" + forms[0]] = float(
+            np.linalg.norm(images[mode] - ref) / np.linalg.norm(ref))
+    return {"target_objective": target, "forms": out}
 
 
 def main() -> int:
@@ -158,9 +207,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--adjoint", default="f16x3", choices=list(PEAK_TFLOPS))
-    ap.add_argument("--workload", type=int, default=1, help="1: 1024^2/1.76M vis, 0: 256^2/36k")
+    ap.add_argument("--workload", type=int, default=1,
+                    help="1: 1024^2/1.76M vis, 0: 256^2/36k, 3: multi-frequency 512^2 I_nu_0 + "
+                         "alpha, 4 spw x 64 channels, ~61M vis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--time-to-image", type=int, default=0, metavar="ITERS",
+                    help="instead of evals/s: L-BFGS time-to-image of the split-precision "
+                         "adjoint against the plain FP32 one at equal objective value")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size (for counter scaling)")
     args = ap.parse_args()
@@ -170,12 +224,26 @@ def main() -> int:
         return 2
 
     from gpuvmem_amd.device import Context, Objective, fw_config
-    from gpuvmem_amd.problem import make_problem
+    from gpuvmem_amd.problem import make_mfs_problem, make_problem
 
     t0 = time.perf_counter()
-    p = make_problem(args.workload)
-    print(f"# workload {args.workload}: {p.M}x{p.N}, {p.nvis} visibilities "
+    p = make_mfs_problem() if args.workload == 3 else make_problem(args.workload)
+    print(f"# workload {args.workload}: {p.M}x{p.N}, {len(p.all_slices())} slices, "
+          f"{p.nvis_total} visibilities "
           f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    if args.time_to_image > 0:
+        r = time_to_image(p, args.time_to_image)
+        f32, f16 = r["forms"]["f32"], r["forms"]["f16x3"]
+        speed = (f32["seconds_to_target"] / f16["seconds_to_target"]
+                 if f16["seconds_to_target"] else None)
+        print(json.dumps({
+            "metric": "time-to-image vs plain FP32 at equal objective (1024^2, 1.76M vis)",
+            "value": speed, "unit": "x (plain FP32 seconds / split-precision seconds)",
+            "higher_is_better": True, "n_gpus": 1, "data": "synthetic",
+            "config": {"workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, chi2+entropy+TSV, "
+                                   f"L-BFGS m=10, {args.time_to_image} iterations of the FP32 form"},
+            **r}), flush=True)
+        return 0
     ctx = Context(p, adjoint=args.adjoint)
     terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
     obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
@@ -199,12 +267,15 @@ def main() -> int:
     print(f"# {args.steps} steps in {elapsed:.3f} s; adjoint kernel {adj_ms:.3f} ms avg over "
           f"{adj_n} launches", file=sys.stderr, flush=True)
 
-    traffic, traffic_src = profiled_traffic(args.adjoint, p.nvis)
-    flops = 4.0 * p.M * p.N * p.nvis
+    nvt = p.nvis_total
+    nsl = len(p.all_slices())
+    traffic, traffic_src = profiled_traffic(args.adjoint, nvt)
+    flops = 4.0 * p.M * p.N * nvt
     achieved = flops / (adj_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.adjoint]
     line = {
-        "metric": "obj+grad evals/s (1024^2, 1.76M vis)",
+        "metric": ("obj+grad evals/s (1024^2, 1.76M vis)" if args.workload == 1 else
+                   f"obj+grad evals/s ({p.M}^2, {nsl} slices, {nvt} vis)"),
         "value": args.steps / elapsed,
         "unit": "evals/s",
         "n_gpus": 1,
@@ -215,12 +286,14 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": DTYPE[args.adjoint],
-        "data": "synthetic (seeded generator: 50-antenna array, 200 Gaussian components, 1 mJy noise)",
+        "data": ("synthetic (seeded generator: 50-antenna array, 200 Gaussian components, 1 mJy "
+                 "noise)" if args.workload == 1 else "synthetic (seeded generator)"),
         "config": {
-            "workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, one channel, "
-                        "chi2+entropy+TSV, CG/L-BFGS objective step",
+            "workload": (f"{p.M}x{p.N} image, {nvt} visibilities, {nsl} channel(s), "
+                         "chi2+entropy+TSV, CG/L-BFGS objective step"),
             "image": [p.M, p.N],
-            "visibilities": p.nvis,
+            "visibilities": nvt,
+            "slices": nsl,
             "adjoint": args.adjoint,
         },
         "roofline": {

@@ -43,6 +43,7 @@ typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
 enum { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2 };
 
@@ -58,30 +59,6 @@ __device__ __forceinline__ void sincos_turns(float t, float* s, float* c) {
 }
 
 __device__ __forceinline__ float fract(float x) { return __builtin_amdgcn_fractf(x); }
-
-__device__ __forceinline__ unsigned short bf16_rne(float x) {
-  unsigned int u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f(unsigned short h) {
-  return __uint_as_float(((unsigned int)h) << 16);
-}
-
-// Splits x into hi + lo in the 16-bit format of PREC.
-template <int PREC>
-__device__ __forceinline__ void split16(float x, unsigned short* hi, unsigned short* lo) {
-  if (PREC == PREC_F16) {
-    _Float16 h = (_Float16)x;
-    _Float16 l = (_Float16)(x - (float)h);
-    *hi = __builtin_bit_cast(unsigned short, h);
-    *lo = __builtin_bit_cast(unsigned short, l);
-  } else {
-    unsigned short h = bf16_rne(x);
-    *hi = h;
-    *lo = bf16_rne(x - bf16_to_f(h));
-  }
-}
 
 __device__ __forceinline__ f32x16_t mfma16(const u16x8_t& a, const u16x8_t& b, f32x16_t c,
                                            int prec) {
@@ -119,15 +96,23 @@ struct AdjLds<PREC_F32> {
   float4 rec_d[2][kKV];
 };
 
+// Byte offset of 16-byte group g of a row, with an XOR swizzle of the group
+// by row bits.  For 64-byte rows (NG = 4) the pattern (row ^ row >> 2) & 3
+// was measured free of LDS bank conflicts (SQ_LDS_BANK_CONFLICT = 0) for
+// both access patterns of this kernel: 16-byte reads of 32 consecutive rows
+// per half-wave (MFMA operands) and 16-byte writes of 64 consecutive rows
+// per wave (generation).
 template <int NG>
 __device__ __forceinline__ int swz(int row, int g) {
-  return row * (NG * 16) + ((g ^ ((row >> 2) & (NG - 1))) << 4);
+  const int f = NG == 4 ? (row ^ (row >> 2)) & 3 : (row ^ (row >> 3)) & 7;
+  return row * (NG * 16) + ((g ^ f) << 4);
 }
 
-// hi/lo split of a pair of floats into packed 16-bit pairs.  FP16: hi is x
-// truncated to FP16 (RTZ conversion; for normal FP16 values that is x with
-// the low 13 mantissa bits cleared), lo = x - hi exactly in FP32, then
-// converted.  BF16: round to nearest even for both parts.
+// hi/lo split of a pair of floats into packed 16-bit pairs.
+// FP16: hi = x truncated to FP16 (v_cvt_pkrtz; for normal FP16 values that is
+// x with the low 13 mantissa bits cleared), lo = x - hi exact in FP32, then
+// converted.  BF16: both parts by the hardware round-to-nearest-even
+// conversion; the FP32 value of a BF16 number is its bits shifted up by 16.
 template <int PREC>
 __device__ __forceinline__ void split_pair(float x0, float x1, unsigned int* hi, unsigned int* lo) {
   if constexpr (PREC == PREC_F16) {
@@ -136,16 +121,20 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned int* hi,
     *hi = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(x0, x1));
     *lo = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(x0 - h0, x1 - h1));
   } else {
-    const unsigned short a = bf16_rne(x0), b = bf16_rne(x1);
-    *hi = (unsigned int)a | ((unsigned int)b << 16);
-    const unsigned short c = bf16_rne(x0 - bf16_to_f(a)), d = bf16_rne(x1 - bf16_to_f(b));
-    *lo = (unsigned int)c | ((unsigned int)d << 16);
+    bf16x2_t h;
+    h.x = (__bf16)x0;
+    h.y = (__bf16)x1;
+    const unsigned int hb = __builtin_bit_cast(unsigned int, h);
+    bf16x2_t l;
+    l.x = (__bf16)(x0 - __uint_as_float(hb << 16));
+    l.y = (__bf16)(x1 - __uint_as_float(hb & 0xffff0000u));
+    *hi = hb;
+    *lo = __builtin_bit_cast(unsigned int, l);
   }
 }
 
 constexpr int kGemmThreads = 512;
 constexpr int kPScale = 15;  // max |P| = 2^15 < 65504 (FP16 max)
-constexpr int kQScale = 14;
 constexpr int kHalfKV = kKV / 2;
 
 // One workgroup (8 waves, 2 per SIMD) owns a 256x256 tile of G for one
@@ -164,25 +153,32 @@ constexpr int kHalfKV = kKV / 2;
 // scale per value.
 template <int PREC>
 __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
-    const float4* __restrict__ phs, const float* __restrict__ wcoord,
-    const float2* __restrict__ cpol, long nvis, const float2* __restrict__ xtab,
-    const float2* __restrict__ ytab, int N, int tiles_x, int ntiles, int nsplit, long kper,
-    const unsigned int* __restrict__ cmax, float* __restrict__ part) {
+    const SliceDev* __restrict__ tab, const int4* __restrict__ splits,
+    const float2* __restrict__ xtab, const float2* __restrict__ ytab, int N, int tiles_x,
+    int ntiles, const unsigned int* __restrict__ cmaxs, float* __restrict__ part) {
   __shared__ AdjLds<PREC> sm;
   constexpr int NG = AdjLds<PREC>::kGroups;
 
   // XCD-aware order: the workgroups of one K-range (which read the same
   // visibilities) are dispatched to the same XCD (blockIdx % 8) and share its L2.
-  const int nwg = ntiles * nsplit;
+  const int nwg = gridDim.x;
   const int per_xcd = nwg / 8;
   const int lin = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   const int split = lin / ntiles;
   const int tile = lin % ntiles;
   const int ti = tile / tiles_x, tj = tile % tiles_x;
 
-  const long kbeg = (long)split * kper;
-  const long kend = min(nvis, kbeg + kper);
+  // this workgroup's visibility range: slice e.x, [e.y, e.z)
+  const int4 e = splits[split];
+  const SliceDev& sd = tab[e.x];
+  const float4* __restrict__ phs = sd.phs;
+  const float* __restrict__ wcoord = sd.wcoord;
+  const float2* __restrict__ cpol = sd.cpol;
+  const long nvis = sd.nvis;
+  const long kbeg = e.y;
+  const long kend = e.z;
   const int nchunks = kend > kbeg ? (int)((kend - kbeg + kKV - 1) / kKV) : 0;
+  const unsigned int* __restrict__ cmax = cmaxs + e.x;
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -195,12 +191,11 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
   const float2 xt = cvalid ? xtab[col] : make_float2(0.f, 0.f);
 
   const float cm = __uint_as_float(*cmax);
-  // Operand scaling (powers of two, undone in the epilogue): |P| <= 2^kPScale
-  // and |Q| <= 2^kQScale keep the lo parts of the FP16 split in the normal
-  // range instead of the subnormals.
+  // P is scaled by a power of two (undone in the epilogue) so |P| <= 2^kPScale
+  // fits FP16; the scale is folded into the visibility amplitudes as they
+  // are staged.  Rows and columns past the image produce finite values that
+  // only reach outputs past the image, which are not written.
   const float scale = cm > 0.0f ? exp2f((float)kPScale - ceilf(log2f(cm))) : 1.0f;
-  const float rscale = rvalid ? scale : 0.0f;  // P of rows past the image is 0
-  constexpr float qscale = (float)(1 << kQScale);
 
   const int lr = lane & 31, lh = lane >> 5;
   const int wr = (wave & 1) * 128, wc = (wave >> 1) * 64;
@@ -212,7 +207,7 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
     const long kc = min(k, nvis - 1);
     s = phs[kc];
     const float2 p = cpol[kc];
-    d = make_float4(k < kend ? p.x : 0.0f, p.y, wcoord[kc], 0.f);
+    d = make_float4(k < kend ? p.x * scale : 0.0f, p.y, wcoord[kc], 0.f);
   };
   auto store_rec = [&](int buf, const float4& s, const float4& d) {
     sm.rec_s[buf][slot] = s;
@@ -230,11 +225,10 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
       float sy, cy, sx, cx;
       sincos_turns(fract(s.z * yt.x) + fmaf(s.w, yt.x, fmaf(d.z, yt.y, d.y)), &sy, &cy);
       sincos_turns(fract(s.x * xt.x) + fmaf(s.y, xt.x, d.z * xt.y), &sx, &cx);
-      const float amp = d.x * rscale;
-      pv[2 * q] = amp * cy;
-      pv[2 * q + 1] = -(amp * sy);
-      qv[2 * q] = cx * qscale;
-      qv[2 * q + 1] = sx * qscale;
+      pv[2 * q] = d.x * cy;
+      pv[2 * q + 1] = -(d.x * sy);
+      qv[2 * q] = cx;
+      qv[2 * q + 1] = sx;
     }
     if constexpr (PREC == PREC_F32) {
 #pragma unroll
@@ -416,65 +410,146 @@ __global__ __launch_bounds__(256) void adjoint_direct_kernel(
   if (valid) part[pix] = (float)acc;
 }
 
-// Sum of the split-K partials (FP64), -fg PB on the mask, 1/nvis if
-// normalize, then the chain rule into result.
-__global__ void adjoint_epilogue_kernel(const float* __restrict__ part, int nsplit, long n,
-                                        const unsigned int* __restrict__ cmax, int scaled,
+// Per pixel inside the mask and per slice in slice order: the FP64 sum of
+// the slice's split-K partials, times -fg PB (and 1/nvis if normalize), then
+// the chain rule of DChi2_total_I_nu_0 / DChi2_total_alpha accumulated into
+// result in float, one slice after the other as the reference's per-channel
+// launches do (src/functions.cu:4456-4558).
+__global__ void adjoint_epilogue_kernel(const float* __restrict__ part,
+                                        const int* __restrict__ p0,
+                                        const SliceDev* __restrict__ tab, int nsl, long n,
+                                        const unsigned int* __restrict__ cmaxs, int scaled,
                                         const float* __restrict__ noise, float noise_cut,
-                                        const float* __restrict__ pb, float fg_scale,
-                                        double inv_nvis, float nudiv, float threshold,
+                                        float fg_scale, int normalize, float threshold,
                                         int flag_opt, const float* __restrict__ I,
                                         float* __restrict__ result) {
-  double inv_scale = 1.0;
-  if (scaled) {
-    const float cm = __uint_as_float(*cmax);
-    inv_scale = (cm > 0.0f ? (double)exp2f(ceilf(log2f(cm)) - (float)kPScale) : 1.0) *
-                exp2(-(double)kQScale);
-  }
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
        k += (long)gridDim.x * blockDim.x) {
     if (!(noise[k] < noise_cut)) continue;
-    // FP64 sum over the splits in split order; eight loads are issued before
-    // their adds so each thread keeps several HBM requests in flight.
-    double g = 0.0;
-    int s = 0;
-    for (; s + 8 <= nsplit; s += 8) {
-      float v[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = part[(long)(s + u) * n + k];
-#pragma unroll
-      for (int u = 0; u < 8; u++) g += (double)v[u];
-    }
-    for (; s < nsplit; s++) g += (double)part[(long)s * n + k];
-    // DChi2: dchi2 = -(sum * fg_scale * atten) [/ nvis]
-    const float d = (float)(-(g * inv_scale) * (double)(fg_scale * pb[k]) * inv_nvis);
     const float i0 = I[k], alpha = I[n + k];
-    const float dI = powf(nudiv, alpha);
-    if (flag_opt % 2 == 0) {
-      result[k] += d * dI;
-    } else if (i0 > threshold) {
-      result[n + k] += d * (i0 * dI * fg_scale * logf(nudiv));
+    float r0 = result[k], r1 = result[n + k];
+    for (int sl = 0; sl < nsl; sl++) {
+      // FP64 sum over the slice's splits in split order; eight loads are
+      // issued before their adds so each thread keeps several HBM requests
+      // in flight.
+      const int pa = p0[sl], pe = p0[sl + 1];
+      double g = 0.0;
+      int s = pa;
+      for (; s + 8 <= pe; s += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = part[(long)(s + u) * n + k];
+#pragma unroll
+        for (int u = 0; u < 8; u++) g += (double)v[u];
+      }
+      for (; s < pe; s++) g += (double)part[(long)s * n + k];
+      double inv_scale = 1.0;
+      if (scaled) {
+        const float cm = __uint_as_float(cmaxs[sl]);
+        inv_scale = cm > 0.0f ? (double)exp2f(ceilf(log2f(cm)) - (float)kPScale) : 1.0;
+      }
+      const SliceDev& sd = tab[sl];
+      const double inv_nvis = normalize ? 1.0 / (double)sd.nvis : 1.0;
+      // DChi2: dchi2 = -(sum * fg_scale * atten) [/ nvis]
+      const float d = (float)(-(g * inv_scale) * (double)(fg_scale * sd.pb[k]) * inv_nvis);
+      const float dI = powf(sd.nudiv, alpha);
+      if (flag_opt % 2 == 0) {
+        r0 += d * dI;
+      } else if (i0 > threshold) {
+        r1 += d * (i0 * dI * fg_scale * logf(sd.nudiv));
+      }
     }
+    if (flag_opt % 2 == 0)
+      result[k] = r0;
+    else
+      result[n + k] = r1;
   }
 }
 
 namespace {
 
-int choose_nsplit(long nvis, int ntiles, int cu_count, long n_pixels) {
-  // Each split accumulates its visibilities in FP32 MFMA accumulators; the
-  // splits are combined in FP64.  The FP32 rounding error grows with the
-  // square root of the accumulation length, so splits are kept near kSplitVis
-  // visibilities (about 10^3 accumulation steps), within a workspace of
-  // kMaxWorkspace bytes, and at least enough splits to fill the GPU.
+// Splits the visibilities of every slice into ranges for the GEMM
+// workgroups.  Each range accumulates in FP32 MFMA accumulators and the
+// ranges are combined in FP64, so ranges are kept near kSplitVis
+// visibilities (about 10^3 accumulation steps: the FP32 rounding error grows
+// with the square root of the accumulation length), with at least enough
+// ranges to fill the GPU, within a workspace of at most kMaxWorkspace bytes
+// (and half the free memory); the total is padded to a multiple of 8 for
+// the XCD remap with empty ranges.  The direct form takes one range per
+// slice.
+gvm_status build_splits(gvm_ctx* ctx, int mode, int ntiles, long n) {
   constexpr long kSplitVis = 3072;
-  constexpr long kMaxWorkspace = 4L << 30;
-  long ns = (nvis + kSplitVis - 1) / kSplitVis;
-  const long fill = (4L * std::max(cu_count, 64) + ntiles - 1) / ntiles;
-  ns = std::max(ns, std::min(fill, std::max(1L, nvis / 512)));
-  ns = std::min(ns, std::max(8L, kMaxWorkspace / (long)(sizeof(float) * n_pixels)));
-  ns = std::max(ns, 1L);
-  ns = ((ns + 7) / 8) * 8;  // the XCD remap needs multiples of 8 workgroups
-  return (int)std::min(ns, 4096L);
+  constexpr long kMaxWorkspace = 16L << 30;
+  const int nsl = (int)ctx->slices.size();
+  std::vector<int4> tab;
+  std::vector<int> p0(nsl + 1, 0);
+  if (mode == GVM_ADJOINT_DIRECT) {
+    for (int s = 0; s < nsl; s++) {
+      p0[s] = s;
+      tab.push_back(make_int4(s, 0, (int)ctx->slices[s].nvis, 0));
+    }
+    p0[nsl] = nsl;
+  } else {
+    size_t free_b = 0, total_b = 0;
+    GVM_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const long avail = (long)(free_b + sizeof(float) * ctx->adj_part_elems) / 2;
+    const long budget = std::max<long>(8, std::min(kMaxWorkspace, avail) / (long)(sizeof(float) * n));
+    const long fill = (4L * std::max(ctx->cu_count, 64) + ntiles - 1) / ntiles;
+    long kvis = kSplitVis;
+    std::vector<long> ns(nsl);
+    for (;;) {
+      long total = 0;
+      for (int s = 0; s < nsl; s++) {
+        const long nv = ctx->slices[s].nvis;
+        long k = (nv + kvis - 1) / kvis;
+        // small observations: more, shorter ranges so the GPU fills
+        k = std::max(k, std::min((fill + nsl - 1) / nsl, std::max(1L, nv / 512)));
+        ns[s] = std::max(1L, k);
+        total += ns[s];
+      }
+      if (((total + 7) / 8) * 8 <= budget || kvis >= (1L << 30)) break;
+      kvis *= 2;
+    }
+    for (int s = 0; s < nsl; s++) {
+      const long nv = ctx->slices[s].nvis;
+      const long kper = ((nv + ns[s] - 1) / ns[s] + kKV - 1) / kKV * kKV;
+      p0[s] = (int)tab.size();
+      for (long j = 0; j < ns[s]; j++) {
+        const long kb = std::min(nv, j * kper), ke = std::min(nv, kb + kper);
+        tab.push_back(make_int4(s, (int)kb, (int)ke, 0));
+      }
+    }
+    while (tab.size() % 8) {
+      const int last = nsl - 1;
+      const int nv = (int)ctx->slices[last].nvis;
+      tab.push_back(make_int4(last, nv, nv, 0));  // empty range: a zero partial tile
+    }
+    p0[nsl] = (int)tab.size();
+  }
+  if ((int)tab.size() > ctx->split_cap) {
+    GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->split_tab) GVM_HIP_TRY(hipFree(ctx->split_tab));
+    ctx->split_tab = nullptr;
+    ctx->split_cap = 0;
+    GVM_HIP_TRY(hipMalloc((void**)&ctx->split_tab, sizeof(int4) * tab.size()));
+    ctx->split_cap = (int)tab.size();
+  }
+  GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  GVM_HIP_TRY(hipMemcpy(ctx->split_tab, tab.data(), sizeof(int4) * tab.size(),
+                        hipMemcpyHostToDevice));
+  GVM_HIP_TRY(hipMemcpy(ctx->slice_p0, p0.data(), sizeof(int) * p0.size(),
+                        hipMemcpyHostToDevice));
+  ctx->nsplit_total = (int)tab.size();
+  ctx->split_mode = mode;
+  const size_t need = (size_t)ctx->nsplit_total * n;
+  if (need > ctx->adj_part_elems) {
+    if (ctx->adj_part) GVM_HIP_TRY(hipFree(ctx->adj_part));
+    ctx->adj_part = nullptr;
+    ctx->adj_part_elems = 0;
+    GVM_HIP_TRY(hipMalloc((void**)&ctx->adj_part, sizeof(float) * need));
+    ctx->adj_part_elems = need;
+  }
+  return GVM_OK;
 }
 
 }  // namespace
@@ -486,57 +561,50 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
   GVM_HIP_TRY(hipSetDevice(ctx->device));
   const int tiles_x = (N + kTile - 1) / kTile;
   const int ntiles = tiles_x * tiles_x;
-  for (size_t si = 0; si < ctx->slices.size(); si++) {
-    const Slice& s = ctx->slices[si];
-    const int mode = ctx->adjoint_mode;
-    const int nsplit = mode == GVM_ADJOINT_DIRECT ? 1 : choose_nsplit(s.nvis, ntiles, ctx->cu_count, n);
-    const size_t need = (size_t)nsplit * n;
-    if (need > ctx->adj_part_elems) {
-      GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
-      if (ctx->adj_part) GVM_HIP_TRY(hipFree(ctx->adj_part));
-      ctx->adj_part = nullptr;
-      ctx->adj_part_elems = 0;
-      GVM_HIP_TRY(hipMalloc((void**)&ctx->adj_part, sizeof(float) * need));
-      ctx->adj_part_elems = need;
-    }
-    const long kper = (((s.nvis + nsplit - 1) / nsplit) + kKV - 1) / kKV * kKV;
-    GVM_HIP_TRY(hipEventRecord(ctx->ev_a, ctx->stream));
-    if (mode == GVM_ADJOINT_DIRECT) {
-      const double dxr = g.deltax * 3.14159265358979323846 / 180.0;
-      const double dyr = g.deltay * 3.14159265358979323846 / 180.0;
+  const int mode = ctx->adjoint_mode;
+  const int nsl = (int)ctx->slices.size();
+  if (ctx->split_mode != mode) {
+    gvm_status st = build_splits(ctx, mode, ntiles, n);
+    if (st) return st;
+  }
+  GVM_HIP_TRY(hipEventRecord(ctx->ev_a, ctx->stream));
+  if (mode == GVM_ADJOINT_DIRECT) {
+    const double dxr = g.deltax * 3.14159265358979323846 / 180.0;
+    const double dyr = g.deltay * 3.14159265358979323846 / 180.0;
+    for (int si = 0; si < nsl; si++) {
+      const Slice& s = ctx->slices[si];
       hipLaunchKernelGGL(adjoint_direct_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                          ctx->stream, s.phs, s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab,
-                         ctx->ytab, N, dxr, dyr, ctx->adj_part);
-    } else {
-      const dim3 grid(ntiles * nsplit);
-      if (mode == GVM_ADJOINT_F32)
-        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F32>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
-                           s.wcoord, s.cpol, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
-                           ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
-      else if (mode == GVM_ADJOINT_F16X3)
-        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F16>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
-                           s.wcoord, s.cpol, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
-                           ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
-      else
-        hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_BF16>, grid, dim3(kGemmThreads), 0, ctx->stream, s.phs,
-                           s.wcoord, s.cpol, s.nvis, ctx->xtab, ctx->ytab, N, tiles_x,
-                           ntiles, nsplit, kper, ctx->cmax + si, ctx->adj_part);
+                         ctx->ytab, N, dxr, dyr, ctx->adj_part + (long)si * n);
+      GVM_CHECK_LAUNCH();
     }
+  } else {
+    const dim3 grid(ntiles * ctx->nsplit_total);
+    if (mode == GVM_ADJOINT_F32)
+      hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F32>, grid, dim3(kGemmThreads), 0, ctx->stream,
+                         ctx->slice_tab, ctx->split_tab, ctx->xtab, ctx->ytab, N, tiles_x, ntiles,
+                         ctx->cmax, ctx->adj_part);
+    else if (mode == GVM_ADJOINT_F16X3)
+      hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_F16>, grid, dim3(kGemmThreads), 0, ctx->stream,
+                         ctx->slice_tab, ctx->split_tab, ctx->xtab, ctx->ytab, N, tiles_x, ntiles,
+                         ctx->cmax, ctx->adj_part);
+    else
+      hipLaunchKernelGGL(adjoint_gemm_kernel<PREC_BF16>, grid, dim3(kGemmThreads), 0, ctx->stream,
+                         ctx->slice_tab, ctx->split_tab, ctx->xtab, ctx->ytab, N, tiles_x, ntiles,
+                         ctx->cmax, ctx->adj_part);
     GVM_CHECK_LAUNCH();
-    GVM_HIP_TRY(hipEventRecord(ctx->ev_b, ctx->stream));
-    const double inv_nvis = normalize ? 1.0 / (double)s.nvis : 1.0;
-    hipLaunchKernelGGL(adjoint_epilogue_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       ctx->stream,
-                       ctx->adj_part, nsplit, n, ctx->cmax + si,
-                       mode == GVM_ADJOINT_DIRECT ? 0 : 1, ctx->noise, g.noise_cut, s.pb,
-                       g.fg_scale, inv_nvis, s.freq / g.nu_0, g.threshold, flag_opt, I, result);
-    GVM_CHECK_LAUNCH();
-    GVM_HIP_TRY(hipEventSynchronize(ctx->ev_b));
-    float ms = 0.f;
-    GVM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
-    ctx->adj_ms_total += ms;
-    ctx->adj_count += 1;
   }
+  GVM_HIP_TRY(hipEventRecord(ctx->ev_b, ctx->stream));
+  hipLaunchKernelGGL(adjoint_epilogue_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     ctx->stream, ctx->adj_part, ctx->slice_p0, ctx->slice_tab, nsl, n, ctx->cmax,
+                     mode == GVM_ADJOINT_DIRECT ? 0 : 1, ctx->noise, g.noise_cut, g.fg_scale,
+                     normalize, g.threshold, flag_opt, I, result);
+  GVM_CHECK_LAUNCH();
+  GVM_HIP_TRY(hipEventSynchronize(ctx->ev_b));
+  float ms = 0.f;
+  GVM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
+  ctx->adj_ms_total += ms;
+  ctx->adj_count += 1;
   return GVM_OK;
 }
 

@@ -217,6 +217,9 @@ extern "C" gvm_status gvm_destroy(gvm_ctx* ctx) {
   (void)hipFree(ctx->dscalars);
   (void)hipFree(ctx->cmax);
   (void)hipFree(ctx->adj_part);
+  (void)hipFree(ctx->slice_tab);
+  (void)hipFree(ctx->split_tab);
+  (void)hipFree(ctx->slice_p0);
   if (ctx->hscalars) (void)hipHostFree(ctx->hscalars);
   if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
   if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
@@ -249,6 +252,9 @@ static gvm_status create_impl(const gvm_geometry* g, const float* noise_host, gv
   gvm_status st = upload(&ctx->noise, noise);
   if (st) return st;
   if ((st = dev_alloc(&ctx->grid, n))) return st;
+  ctx->grid_slices = 1;
+  if ((st = dev_alloc(&ctx->slice_tab, kMaxSlots))) return st;
+  if ((st = dev_alloc(&ctx->slice_p0, kMaxSlots + 1))) return st;
 
   // e^{+2 pi i k / N} (the CUFFT_INVERSE sign of the reference FFT)
   std::vector<float2> tw(N);
@@ -411,8 +417,38 @@ extern "C" gvm_status gvm_add_visibilities(gvm_ctx* ctx, float freq, long nvis,
   }
   GVM_HIP_TRY(hipMemset(s.vr, 0, sizeof(float2) * nvis));
   GVM_HIP_TRY(hipMemset(s.cpol, 0, sizeof(float2) * nvis));
+  // one uv grid per slice for the batched forward pass
+  const size_t need = ctx->slices.size() + 1;
+  if (need > ctx->grid_slices) {
+    const size_t cap = std::max(need, 2 * ctx->grid_slices);
+    float2* grid = nullptr;
+    GVM_HIP_TRY(hipDeviceSynchronize());
+    if (hipMalloc((void**)&grid, sizeof(float2) * (size_t)g.M * g.N * cap) != hipSuccess) {
+      free_slice(s);
+      set_error("gvm_add_visibilities: out of device memory for the uv grids");
+      return GVM_ERR_OOM;
+    }
+    (void)hipFree(ctx->grid);
+    ctx->grid = grid;
+    ctx->grid_slices = cap;
+  }
+  SliceDev d;
+  d.cell = s.cell;
+  d.frac = s.frac;
+  d.vo = s.vo;
+  d.weight = s.weight;
+  d.vr = s.vr;
+  d.cpol = s.cpol;
+  d.phs = s.phs;
+  d.wcoord = s.wcoord;
+  d.pb = s.pb;
+  d.nvis = s.nvis;
+  d.nudiv = freq / g.nu_0;
+  d.pad = 0.f;
+  GVM_HIP_TRY(hipMemcpy(ctx->slice_tab + ctx->slices.size(), &d, sizeof(d), hipMemcpyHostToDevice));
   GVM_HIP_TRY(hipDeviceSynchronize());
   ctx->slices.push_back(s);
+  ctx->split_mode = -1;
   ctx->have_residuals = false;
   if (slice_out) *slice_out = (int)ctx->slices.size() - 1;
   return GVM_OK;

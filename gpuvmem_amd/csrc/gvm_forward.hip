@@ -3,7 +3,9 @@
 // Per slice (channel x stokes) the reference runs calculateInu, apply_beam2I,
 // cuFFT C2C (inverse, unnormalised), phase_rotate, vis_mod, residual and
 // chi2Vector + a float tree reduction as eight separate launches with a device
-// synchronisation after each (src/functions.cu:4323-4454).  Here:
+// synchronisation after each (src/functions.cu:4323-4454).  Here every step
+// covers all slices of the context in one launch (blockIdx.y = slice, one uv
+// grid per slice, per-slice descriptors in a device table):
 //
 //  1. clip_kernel        clip2IWNoise in place (once per evaluation);
 //  2. fft_pass<rows>     I_nu = clip(I0 (nu/nu0)^alpha), times PB and fg_scale,
@@ -20,6 +22,8 @@
 // rotation e^{-2 pi i (u x0/M + v y0/N)} (src/functions.cu:2483-2518)
 // factorises per axis, so each pass applies its own factor to its output.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "gvm_internal.h"
 
@@ -149,16 +153,20 @@ __device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ 
   }
 }
 
-// Row pass: build I_nu * PB * fg_scale for B rows (calculateInu +
-// apply_beam2I, src/functions.cu:3939-3966, 2424-2444), transform each row,
-// multiply by rot_x[column frequency], write the rows of the grid.
+// Row pass: build I_nu * PB * fg_scale for B rows of slice blockIdx.y
+// (calculateInu + apply_beam2I, src/functions.cu:3939-3966, 2424-2444),
+// transform each row, multiply by rot_x[column frequency], write the rows of
+// that slice's grid.
 template <int N, int B>
 __global__ __launch_bounds__(256) void fft_rows_kernel(
-    const float* __restrict__ I, const float* __restrict__ pb, float fg_scale, float nudiv,
+    const float* __restrict__ I, const SliceDev* __restrict__ tab, float fg_scale,
     float floor_v, const float2* __restrict__ tw, const float2* __restrict__ rot_x,
-    float2* __restrict__ grid) {
+    float2* __restrict__ grids) {
   __shared__ float2 lds[B * N];
   const long n = (long)N * N;
+  const float* __restrict__ pb = tab[blockIdx.y].pb;
+  const float nudiv = tab[blockIdx.y].nudiv;
+  float2* __restrict__ grid = grids + (long)blockIdx.y * n;
   const int row0 = blockIdx.x * B;
   for (int idx = threadIdx.x; idx < B * N; idx += 256) {
     const long k = (long)(row0 + idx / N) * N + (idx % N);
@@ -174,12 +182,13 @@ __global__ __launch_bounds__(256) void fft_rows_kernel(
   }
 }
 
-// Column pass: B adjacent columns per block, in place on the grid.
+// Column pass: B adjacent columns per block of slice blockIdx.y, in place.
 template <int N, int B>
-__global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid,
+__global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grids,
                                                        const float2* __restrict__ tw,
                                                        const float2* __restrict__ rot_y) {
   __shared__ float2 lds[B * N];
+  float2* __restrict__ grid = grids + (long)blockIdx.y * N * N;
   const int col0 = blockIdx.x * B;
   for (int idx = threadIdx.x; idx < B * N; idx += 256) {
     const int p = idx / B, t = idx % B;
@@ -194,31 +203,31 @@ __global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grid
 }
 
 template <int N>
-gvm_status launch_fft(gvm_ctx* ctx, const float* I, const float* pb, float nudiv,
-                      float floor_v) {
+gvm_status launch_fft(gvm_ctx* ctx, const float* I, int nsl, float floor_v) {
   constexpr int BR = (4096 / N) > 0 ? (4096 / N) : 1;
   constexpr int BC0 = 8192 / N;
   constexpr int BC = BC0 < 2 ? 2 : (BC0 > N ? N : BC0);
-  hipLaunchKernelGGL((fft_rows_kernel<N, BR>), dim3(N / BR), dim3(256), 0, ctx->stream, I, pb,
-                     ctx->g.fg_scale, nudiv, floor_v, ctx->twiddle, ctx->rot_x, ctx->grid);
+  hipLaunchKernelGGL((fft_rows_kernel<N, BR>), dim3(N / BR, nsl), dim3(256), 0, ctx->stream, I,
+                     ctx->slice_tab, ctx->g.fg_scale, floor_v, ctx->twiddle, ctx->rot_x,
+                     ctx->grid);
   GVM_CHECK_LAUNCH();
-  hipLaunchKernelGGL((fft_cols_kernel<N, BC>), dim3(N / BC), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL((fft_cols_kernel<N, BC>), dim3(N / BC, nsl), dim3(256), 0, ctx->stream,
                      ctx->grid, ctx->twiddle, ctx->rot_y);
   GVM_CHECK_LAUNCH();
   return GVM_OK;
 }
 
-gvm_status model_grid(gvm_ctx* ctx, const float* I, const Slice& s) {
-  const float nudiv = s.freq / ctx->g.nu_0;
+// The phase-rotated uv grids of all slices (ctx->grid, one N x N grid each).
+gvm_status model_grids(gvm_ctx* ctx, const float* I, int nsl) {
   const float floor_v = (float)(-1.0 * (double)ctx->g.eta * (double)ctx->g.minpix);
   switch (ctx->g.N) {
-    case 64: return launch_fft<64>(ctx, I, s.pb, nudiv, floor_v);
-    case 128: return launch_fft<128>(ctx, I, s.pb, nudiv, floor_v);
-    case 256: return launch_fft<256>(ctx, I, s.pb, nudiv, floor_v);
-    case 512: return launch_fft<512>(ctx, I, s.pb, nudiv, floor_v);
-    case 1024: return launch_fft<1024>(ctx, I, s.pb, nudiv, floor_v);
-    case 2048: return launch_fft<2048>(ctx, I, s.pb, nudiv, floor_v);
-    case 4096: return launch_fft<4096>(ctx, I, s.pb, nudiv, floor_v);
+    case 64: return launch_fft<64>(ctx, I, nsl, floor_v);
+    case 128: return launch_fft<128>(ctx, I, nsl, floor_v);
+    case 256: return launch_fft<256>(ctx, I, nsl, floor_v);
+    case 512: return launch_fft<512>(ctx, I, nsl, floor_v);
+    case 1024: return launch_fft<1024>(ctx, I, nsl, floor_v);
+    case 2048: return launch_fft<2048>(ctx, I, nsl, floor_v);
+    case 4096: return launch_fft<4096>(ctx, I, nsl, floor_v);
   }
   set_error("model_grid: unsupported N");
   return GVM_ERR_ARG;
@@ -230,11 +239,18 @@ gvm_status model_grid(gvm_ctx* ctx, const float* I, const Slice& s) {
 // cell 4 + frac 8 + Vo 8 + weight 4 read, Vr 8 + polar c 8 written.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void degrid_chi2_kernel(
-    const float2* __restrict__ grid, int N, const int* __restrict__ cell,
-    const float2* __restrict__ frac, const float2* __restrict__ vo,
-    const float* __restrict__ weight, float2* __restrict__ vr, float2* __restrict__ cpol,
-    long nvis, double* __restrict__ partials, unsigned int* __restrict__ cmax) {
+    const float2* __restrict__ grids, int N, const SliceDev* __restrict__ tab,
+    double* __restrict__ partials, unsigned int* __restrict__ cmaxs) {
   __shared__ double sh[4];
+  const SliceDev& sd = tab[blockIdx.y];
+  const float2* __restrict__ grid = grids + (long)blockIdx.y * N * N;
+  const int* __restrict__ cell = sd.cell;
+  const float2* __restrict__ frac = sd.frac;
+  const float2* __restrict__ vo = sd.vo;
+  const float* __restrict__ weight = sd.weight;
+  float2* __restrict__ vr = sd.vr;
+  float2* __restrict__ cpol = sd.cpol;
+  const long nvis = sd.nvis;
   double acc = 0.0;
   float cm = 0.0f;
   const int mask = N - 1;
@@ -272,8 +288,8 @@ __global__ __launch_bounds__(256) void degrid_chi2_kernel(
   __syncthreads();
   const double mx = block_max((double)cm, sh);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = tot;
-    atomicMax(cmax, __float_as_uint((float)mx));
+    partials[(long)blockIdx.y * kReduceBlocks + blockIdx.x] = tot;
+    atomicMax(cmaxs + blockIdx.y, __float_as_uint((float)mx));
   }
 }
 
@@ -289,16 +305,16 @@ gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize, dou
   hipLaunchKernelGGL(zero_u32, dim3((nsl + 255) / 256), dim3(256), 0, ctx->stream, ctx->cmax,
                      nsl);
   GVM_CHECK_LAUNCH();
-  for (int si = 0; si < nsl; si++) {
-    const Slice& s = ctx->slices[si];
-    gvm_status st = model_grid(ctx, I, s);
-    if (st) return st;
-    hipLaunchKernelGGL(degrid_chi2_kernel, dim3(kReduceBlocks), dim3(256), 0, ctx->stream,
-                       ctx->grid, g.N, s.cell, s.frac, s.vo, s.weight, s.vr, s.cpol, s.nvis,
-                       ctx->partials + (long)si * kReduceBlocks, ctx->cmax + si);
-    GVM_CHECK_LAUNCH();
-  }
-  gvm_status st = finish_reductions(ctx, nsl, kReduceBlocks);
+  gvm_status st = model_grids(ctx, I, nsl);
+  if (st) return st;
+  // blocks per slice: 256 visibilities each, at most kReduceBlocks
+  long vmax = 0;
+  for (const Slice& sl : ctx->slices) vmax = std::max(vmax, sl.nvis);
+  const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, (vmax + 255) / 256));
+  hipLaunchKernelGGL(degrid_chi2_kernel, dim3(nb, nsl), dim3(256), 0, ctx->stream, ctx->grid,
+                     g.N, ctx->slice_tab, ctx->partials, ctx->cmax);
+  GVM_CHECK_LAUNCH();
+  st = finish_reductions(ctx, nsl, nb);
   if (st) return st;
   std::vector<double> sums(nsl);
   st = read_scalars(ctx, nsl, sums.data());

@@ -51,9 +51,26 @@ struct Slice {
   float2* cpol = nullptr;    // (w |Vr|, -arg(Vr) / 2 pi): c = w conj(Vr) in polar form, turns
 };
 
+// What the batched kernels read about one slice (a copy of the pointers of
+// a Slice plus per-slice scalars), kept in a device table indexed by slice.
+struct SliceDev {
+  const int* cell;
+  const float2* frac;
+  const float2* vo;
+  const float* weight;
+  float2* vr;
+  float2* cpol;
+  const float4* phs;
+  const float* wcoord;
+  const float* pb;
+  long nvis;
+  float nudiv;  // freq / nu_0
+  float pad;
+};
+
 constexpr int kReduceBlocks = 1024;
 constexpr int kReduceThreads = 256;
-constexpr int kMaxSlots = 64;
+constexpr int kMaxSlots = 4096;  // slices per context (and reduction slots)
 
 }  // namespace gvm
 
@@ -75,9 +92,18 @@ struct gvm_ctx {
   double* dscalars = nullptr;   // kMaxSlots doubles (device)
   double* hscalars = nullptr;   // kMaxSlots doubles (pinned host)
   unsigned int* cmax = nullptr; // per-slice max |w Vr| (float bits), device
-  // adjoint workspace
+  // batched slices: device descriptor table, uv grids (one per slice)
+  gvm::SliceDev* slice_tab = nullptr;  // kMaxSlots entries
+  size_t grid_slices = 0;              // slices the grid buffer holds
+  // adjoint workspace and its work split: entry p = (slice, kbeg, kend, 0)
+  // of the visibility ranges; slice s owns partial tiles [p0[s], p0[s+1])
   float* adj_part = nullptr;    // split-K partial tiles
   size_t adj_part_elems = 0;
+  int4* split_tab = nullptr;
+  int* slice_p0 = nullptr;      // kMaxSlots + 1 entries
+  int split_cap = 0;
+  int nsplit_total = 0;
+  int split_mode = -1;          // adjoint mode the table was built for (-1: stale)
   int adjoint_mode = GVM_ADJOINT_F16X3;
   std::vector<gvm::Slice> slices;
   bool have_residuals = false;

@@ -123,6 +123,14 @@ extern "C" gvm_status gvm_fw_term_values(gvm_fw_objective* obj, double* values, 
 extern "C" gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimizer,
                                       int total_iterations, double ftol, double gtol, int lbfgs_k,
                                       float* p_dev, gvm_fw_result* res, double* traj, int cap) {
+  return gvm_fw_optimize_timed(obj, optimizer, total_iterations, ftol, gtol, lbfgs_k, p_dev, res,
+                               traj, nullptr, cap);
+}
+
+extern "C" gvm_status gvm_fw_optimize_timed(gvm_fw_objective* obj, const char* optimizer,
+                                            int total_iterations, double ftol, double gtol,
+                                            int lbfgs_k, float* p_dev, gvm_fw_result* res,
+                                            double* traj, double* traj_seconds, int cap) {
   if (!obj || !optimizer || !p_dev || !res || total_iterations < 0) return GVM_ERR_ARG;
   try {
     make_current(obj);
@@ -150,6 +158,10 @@ extern "C" gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimiz
     if (traj) {
       const auto& tr = opt->getTrajectory();
       for (int k = 0; k < (int)tr.size() && k < cap; k++) traj[k] = tr[k];
+    }
+    if (traj_seconds) {
+      const auto& ts = opt->getTrajectorySeconds();
+      for (int k = 0; k < (int)ts.size() && k < cap; k++) traj_seconds[k] = ts[k];
     }
   } catch (const std::exception& e) {
     return fail(e);

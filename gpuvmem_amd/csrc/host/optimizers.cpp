@@ -273,7 +273,7 @@ void ConjugateGradient::optimize() {
   globals().flag_opt = this->flag;
   allocateMemoryGpu();
   testof = of;
-  trajectory.clear();
+  startTrajectory();
   exit_reason = 0;
   if (configured) {
     of->configure(globals().N, globals().M, image->getImageCount());
@@ -294,7 +294,7 @@ void ConjugateGradient::optimize() {
   for (int i = 1; i <= this->total_iterations; i++) {
     this->current_iteration = i;
     linmin(p, xi, &fret, nullptr);
-    trajectory.push_back(fret);
+    recordTrajectory(fret);
     if (2.0 * std::fabs(fret - fp) <= this->ftol * (std::fabs(fret) + std::fabs(fp) + EPS)) {
       finish(1);
       return;
@@ -359,7 +359,7 @@ void LBFGS::optimize() {
   globals().flag_opt = this->flag;
   allocateMemoryGpu();
   testof = of;
-  trajectory.clear();
+  startTrajectory();
   exit_reason = 0;
   if (configured) {
     of->configure(globals().N, globals().M, image->getImageCount());
@@ -383,7 +383,7 @@ void LBFGS::optimize() {
     check(gvm_vec_copy(ctx(), p_old, p, tot), "copy p_old");
     check(gvm_vec_copy(ctx(), xi_old, xi, tot), "copy xi_old");
     linmin(p, xi, &fret, nullptr);
-    trajectory.push_back(fret);
+    recordTrajectory(fret);
     if ((fp - fret) / std::max({std::fabs(fret), std::fabs(fp), 1.0}) <= this->ftol) {
       finish(1);
       return;

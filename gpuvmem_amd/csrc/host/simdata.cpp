@@ -151,6 +151,11 @@ extern "C" int gvm_sim_generate(const gvm_sim_config* c, double* uvw_m,
     s.sigma_pix = k < c->n_gaussians ? rs.uniform(c->sigma_min_pix, c->sigma_max_pix) : 0.0;
     s.alpha = rs.uniform(c->alpha_min, c->alpha_max);
   }
+  // flux at this frequency (the sky image and comps_out keep the nu0 fluxes)
+  std::vector<double> flux_nu(ncomp);
+  for (int k = 0; k < ncomp; k++)
+    flux_nu[k] = c->nu0_hz > 0 ? comps[k].flux * std::pow(c->freq_hz / c->nu0_hz, comps[k].alpha)
+                               : comps[k].flux;
   if (comps_out) std::memcpy(comps_out, comps.data(), sizeof(gvm_sim_component) * ncomp);
   if (ncomps_out) *ncomps_out = ncomp;
 
@@ -185,13 +190,13 @@ extern "C" int gvm_sim_generate(const gvm_sim_config* c, double* uvw_m,
     double cxr = (s.x_pix - x0) * dx, cyr = (s.y_pix - y0) * dy;
     double c2 = cxr * cxr + cyr * cyr;
     if (s.sigma_pix <= 0) {
-      terms[k] = {s.flux * std::exp(-c2 / (2.0 * sig_pb * sig_pb)), 0.0, cxr, cyr};
+      terms[k] = {flux_nu[k] * std::exp(-c2 / (2.0 * sig_pb * sig_pb)), 0.0, cxr, cyr};
     } else {
       double ss = s.sigma_pix * cell;
       double sum2 = ss * ss + sig_pb * sig_pb;
       double sp2 = ss * ss * sig_pb * sig_pb / sum2;
       double shrink = sig_pb * sig_pb / sum2;
-      double amp = s.flux / (2.0 * kPi * ss * ss) * std::exp(-c2 / (2.0 * sum2)) * 2.0 * kPi * sp2;
+      double amp = flux_nu[k] / (2.0 * kPi * ss * ss) * std::exp(-c2 / (2.0 * sum2)) * 2.0 * kPi * sp2;
       terms[k] = {amp, sp2, cxr * shrink, cyr * shrink};
     }
   }
@@ -219,6 +224,29 @@ extern "C" int gvm_sim_generate(const gvm_sim_config* c, double* uvw_m,
   return 0;
 }
 
+extern "C" int gvm_sim_generate_flagged(const gvm_sim_config* c, double* uvw_m, float* vo,
+                                        float* weight, float* sky, gvm_sim_component* comps_out,
+                                        int* ncomps_out, long* nkept) {
+  if (!nkept || c->flag_fraction < 0.0 || c->flag_fraction >= 1.0) return 1;
+  const int rc = gvm_sim_generate(c, uvw_m, vo, weight, sky, comps_out, ncomps_out);
+  if (rc != 0) return rc;
+  const long nvis = gvm_sim_count_visibilities(c);
+  Rng rf(c->seed_flag);
+  long k = 0;
+  for (long i = 0; i < nvis; i++) {
+    if (c->flag_fraction > 0.0 && rf.uniform() < c->flag_fraction) continue;
+    if (k != i) {
+      for (int d = 0; d < 3; d++) uvw_m[3 * k + d] = uvw_m[3 * i + d];
+      vo[2 * k] = vo[2 * i];
+      vo[2 * k + 1] = vo[2 * i + 1];
+      weight[k] = weight[i];
+    }
+    k++;
+  }
+  *nkept = k;
+  return 0;
+}
+
 extern "C" void gvm_sim_default_config(int which, gvm_sim_config* c) {
   std::memset(c, 0, sizeof(*c));
   c->cell_arcsec = 0.05;
@@ -232,7 +260,37 @@ extern "C" void gvm_sim_default_config(int which, gvm_sim_config* c) {
   c->point_flux_max = 1.0;
   c->alpha_min = 0.0;
   c->alpha_max = 0.0;
-  if (which == 1) {
+  if (which == 3) {
+    // One channel of the 512^2 multi-frequency workload: 35 antennas
+    // Gaussian-radial in a 2 km disk, 504 dumps of 10 s (595 x 504 = 299,880
+    // rows); 30 Gaussian components (power-law fluxes) + 5 points with
+    // spectral indices U(-1, 3) about nu0 = 225 GHz.  The caller sets
+    // freq_hz (210-240 GHz), flag_fraction and the per-channel seeds.
+    c->npix = 512;
+    c->n_antennas = 35;
+    c->layout = GVM_SIM_LAYOUT_GAUSSIAN;
+    c->array_diameter_m = 2000.0;
+    c->layout_sigma_m = 500.0;
+    c->ha_start_hours = -0.7;
+    c->ha_end_hours = 0.7;
+    c->dump_seconds = 10.0;
+    c->freq_hz = 225e9;
+    c->nu0_hz = 225e9;
+    c->n_gaussians = 30;
+    c->n_points = 5;
+    c->sigma_min_pix = 1.0;
+    c->sigma_max_pix = 6.0;
+    c->flux_law = GVM_SIM_FLUX_POWERLAW;
+    c->flux_index = -1.5;
+    c->flux_min = 1e-3;
+    c->flux_max = 1.0;
+    c->alpha_min = -1.0;
+    c->alpha_max = 3.0;
+    c->seed_layout = 30;
+    c->seed_sky = 31;
+    c->seed_noise = 32;
+    c->seed_flag = 33;
+  } else if (which == 1) {
     // 1024^2; 50 antennas Gaussian-radial in a 2 km disk; 4 h at 10 s;
     // 200 Gaussian components with power-law fluxes.
     c->npix = 1024;

@@ -67,7 +67,8 @@ class Context:
         self.set_adjoint(adjoint)
         self.n = problem.M * problem.N
         if add_data:
-            self.add(problem.freq, problem.uvw, problem.vo, problem.weight)
+            for f, uvw, vo, w in problem.all_slices():
+                self.add(f, uvw, vo, w)
 
     def add(self, freq: float, uvw: np.ndarray, vo: np.ndarray, weight: np.ndarray) -> int:
         uvw = np.ascontiguousarray(uvw, np.float64)
@@ -215,6 +216,19 @@ class Objective:
                         "gvm_fw_optimize")
         n = min(res.iterations, iterations)
         return res, [traj[k] for k in range(n)]
+
+    def optimize_timed(self, img: DeviceBuffer, optimizer: str = "CG-FRPRMN", iterations: int = 50,
+                       ftol: float = 1e-12, gtol: float = 1e-12, lbfgs_k: int = 0):
+        """optimize() that also returns the wall-clock seconds from the start
+        of the run to each trajectory entry."""
+        res = native.FwResult()
+        m = max(1, iterations)
+        traj, secs = (C.c_double * m)(), (C.c_double * m)()
+        native.check_fw(self.fw.gvm_fw_optimize_timed(self.h, optimizer.encode(), iterations, ftol,
+                                                      gtol, lbfgs_k, img.ptr, C.byref(res), traj,
+                                                      secs, iterations), "gvm_fw_optimize_timed")
+        n = min(res.iterations, iterations)
+        return res, [traj[k] for k in range(n)], [secs[k] for k in range(n)]
 
     def set_flag(self, flag_opt: int) -> None:
         """Optimizer::setFlag for the following calls (schedule flag)."""

@@ -53,6 +53,7 @@ class SimConfig(C.Structure):
         ("alpha_min", C.c_double), ("alpha_max", C.c_double),
         ("noise_sigma", C.c_double),
         ("seed_layout", C.c_ulonglong), ("seed_sky", C.c_ulonglong), ("seed_noise", C.c_ulonglong),
+        ("nu0_hz", C.c_double), ("flag_fraction", C.c_double), ("seed_flag", C.c_ulonglong),
     ]
 
 
@@ -134,6 +135,8 @@ FW_API = {
     "gvm_sim_default_config": ([_I, C.POINTER(SimConfig)], None),
     "gvm_sim_count_visibilities": ([C.POINTER(SimConfig)], C.c_long),
     "gvm_sim_generate": ([C.POINTER(SimConfig), _P, _P, _P, _P, C.POINTER(SimComponent), _IP], _I),
+    "gvm_sim_generate_flagged": ([C.POINTER(SimConfig), _P, _P, _P, _P, C.POINTER(SimComponent),
+                                  _IP, C.POINTER(C.c_long)], _I),
     "gvm_setup_hermitian_lambda": ([_L, _P, _P, _F], None),
     "gvm_setup_beam_noise": ([_I, _FP, C.POINTER(_L), C.POINTER(_P), C.POINTER(_P), _D, _DP, _DP,
                               _DP, _DP, _DP, _DP], _I),
@@ -146,6 +149,8 @@ FW_API = {
     "gvm_fw_gradient": ([_P, _P, _P, _I], _I),
     "gvm_fw_term_values": ([_P, _DP, _I, _IP], _I),
     "gvm_fw_optimize": ([_P, C.c_char_p, _I, _D, _D, _I, _P, C.POINTER(FwResult), _DP, _I], _I),
+    "gvm_fw_optimize_timed": ([_P, C.c_char_p, _I, _D, _D, _I, _P, C.POINTER(FwResult), _DP, _DP,
+                               _I], _I),
     "gvm_fw_set_flag": ([_P, _I], _I),
     "gvm_fw_last_error": ([], C.c_char_p),
     "gvm_fw_registered": ([C.c_char_p, C.c_char_p], _I),

@@ -65,6 +65,13 @@ typedef struct {
 gvm_status gvm_fw_optimize(gvm_fw_objective* obj, const char* optimizer, int total_iterations,
                            double ftol, double gtol, int lbfgs_k, float* p_dev,
                            gvm_fw_result* res, double* traj, int cap);
+/* gvm_fw_optimize that also writes, into traj_seconds (optional, capacity
+ * cap), the wall-clock seconds from the start of the run to each trajectory
+ * entry (for time-to-image measurements). */
+gvm_status gvm_fw_optimize_timed(gvm_fw_objective* obj, const char* optimizer,
+                                 int total_iterations, double ftol, double gtol, int lbfgs_k,
+                                 float* p_dev, gvm_fw_result* res, double* traj,
+                                 double* traj_seconds, int cap);
 
 /* Optimizer::setFlag for the next gvm_fw_optimize / value / gradient: the
  * schedule flag_opt (image flag_opt % 2 is optimised; flag_opt > 0 zeroes

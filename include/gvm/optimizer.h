@@ -3,7 +3,7 @@
 // "CG-LBFGS" (LBFGS).  Tolerances are kept as float like the reference; the
 // line-search and direction scalars are FP64.
 #pragma once
-
+#include <chrono>
 #include <vector>
 
 #include "gvm/image.h"
@@ -39,6 +39,9 @@ class Optimizer {
   // line search, and why the loop ended (0 iteration limit, 1 ftol,
   // 2 gradient tolerance, 3 gg == 0).
   const std::vector<double>& getTrajectory() const { return trajectory; }
+  // Wall-clock seconds from the start of optimize() to each trajectory entry
+  // (the value is on the host then, so the device work is complete).
+  const std::vector<double>& getTrajectorySeconds() const { return trajectory_seconds; }
   int getExitReason() const { return exit_reason; }
   double getFinalValue() const { return final_value; }
 
@@ -51,6 +54,18 @@ class Optimizer {
   int total_iterations;
   int current_iteration = 0;
   std::vector<double> trajectory;
+  std::vector<double> trajectory_seconds;
+  std::chrono::steady_clock::time_point start_time;
+  void startTrajectory() {
+    trajectory.clear();
+    trajectory_seconds.clear();
+    start_time = std::chrono::steady_clock::now();
+  }
+  void recordTrajectory(double value) {
+    trajectory.push_back(value);
+    trajectory_seconds.push_back(
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - start_time).count());
+  }
   int exit_reason = 0;
   double final_value = 0.0;
 };

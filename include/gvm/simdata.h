@@ -37,6 +37,12 @@ typedef struct {
   double alpha_min, alpha_max;
   double noise_sigma;
   unsigned long long seed_layout, seed_sky, seed_noise;
+  /* multi-frequency: component fluxes scale as (freq_hz / nu0_hz)^alpha
+   * (nu0_hz <= 0: no scaling); each row is flagged (dropped) with
+   * probability flag_fraction, drawn from seed_flag. */
+  double nu0_hz;
+  double flag_fraction;
+  unsigned long long seed_flag;
 } gvm_sim_config;
 
 typedef struct {
@@ -46,7 +52,11 @@ typedef struct {
   double alpha;
 } gvm_sim_component;
 
-/* which: 0 = 256^2 / 36,000 vis workload, 1 = 1024^2 / 1,764,000 vis. */
+/* which: 0 = 256^2 / 36,000 vis workload, 1 = 1024^2 / 1,764,000 vis,
+ * 3 = one channel of the 512^2 multi-frequency workload (35 antennas,
+ * 595 baselines x 504 dumps = 299,880 rows per channel, nu0 = 225 GHz,
+ * spectral indices U(-1, 3); the caller sets freq_hz, flag_fraction and the
+ * per-channel seeds). */
 void gvm_sim_default_config(int which, gvm_sim_config* cfg);
 long gvm_sim_count_visibilities(const gvm_sim_config* cfg);
 /* uvw_m: 3*nvis doubles (metres); vo: 2*nvis floats; weight: nvis floats;
@@ -55,6 +65,12 @@ long gvm_sim_count_visibilities(const gvm_sim_config* cfg);
 int gvm_sim_generate(const gvm_sim_config* cfg, double* uvw_m, float* vo,
                      float* weight, float* sky, gvm_sim_component* comps_out,
                      int* ncomps_out);
+/* gvm_sim_generate with flagging: the rows that survive flag_fraction are
+ * packed to the front of uvw_m / vo / weight (sized for
+ * gvm_sim_count_visibilities rows) and their number is written to *nkept. */
+int gvm_sim_generate_flagged(const gvm_sim_config* cfg, double* uvw_m, float* vo,
+                             float* weight, float* sky, gvm_sim_component* comps_out,
+                             int* ncomps_out, long* nkept);
 
 #ifdef __cplusplus
 }

@@ -184,7 +184,8 @@ class Oracle:
         self.beam = OBeam(problem.beam_kind, problem.antenna_diameter, problem.pb_factor,
                           problem.pb_cutoff)
         if slices is None:
-            slices = [(problem.freq, problem.uvw, problem.vo, problem.weight)]
+            slices = (problem.all_slices() if hasattr(problem, "all_slices")
+                      else [(problem.freq, problem.uvw, problem.vo, problem.weight)])
         self._keep = []
         chans = []
         for f, u, v, w in slices:

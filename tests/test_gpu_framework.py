@@ -82,10 +82,12 @@ def test_cg_frprmn_trajectory_workload0():
     """CG-FRPRMN, chi2 + entropy, 50 iterations on the 256^2 / 36,000-vis
     workload.  The line search makes this trajectory chaotic: the FP64 oracle
     itself ends 6.5% away in image and 16% in value after 50 iterations when
-    its function values carry 1e-12 relative noise, so the FP32 device path is
-    held to the oracle's trajectory over the first three iterations (1e-4), to
-    monotone decrease, and to the final value and image within that measured
-    spread."""
+    its function values carry 1e-12 relative noise, and with FP32-level noise
+    (1e-7 relative) its values after iterations 1, 2, 3 move by up to 1.9e-7,
+    9.3e-5, 1.2e-4 (three noise seeds).  The FP32 device path is held to the
+    oracle's trajectory over the first three iterations within those
+    envelopes (1e-6, 2e-4, 5e-4), to monotone decrease, and to the final
+    value and image within the 50-iteration spread."""
     p = make_problem(0)
     print(f"case: CG-FRPRMN 50 it, {p.M}^2, nvis={p.nvis}", flush=True)
     terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001)]
@@ -100,7 +102,7 @@ def test_cg_frprmn_trajectory_workload0():
           f"{res.gradient_calls} g, {res.seconds:.2f} s); oracle f {ores.final_value:.10e}; "
           f"first iterations rel err {terr}; final image rel L2 {ierr:.3e}")
     assert res.iterations == ores.iterations == 50
-    assert np.all(terr < 1e-4)
+    assert np.all(terr < np.array([1e-6, 2e-4, 5e-4]))
     assert np.all(np.diff(traj) <= 0)
     assert abs(res.final_value - ores.final_value) <= 0.2 * abs(ores.final_value)
     assert ierr < 0.1
@@ -141,8 +143,11 @@ def test_mfs_flag_schedule_two_channels():
     """MFS::run's schedule on a two-channel problem with a spectral index:
     CG-FRPRMN with flags 0 (I_nu_0), 1 (alpha), 2, 3 in turn on the same
     image, against the oracle running the same schedule.  Each stage is
-    compared on its first iteration and its final value; alpha moves only in
-    the odd stages, I_nu_0 only in the even ones."""
+    compared on its first iteration (1e-6 for the first stage, which starts
+    from the same image; 2e-2 for the later ones, whose starting images
+    carry the chaotic drift of the stages before) and its final value (5%);
+    I_nu_0 moves only in the even stages; alpha moves in the odd stages and,
+    in the even ones, is only zeroed where the clip of flag > 0 applies."""
     from gpuvmem_amd.device import DeviceBuffer
 
     p = small_problem(npix=64, n_antennas=7, seed=41)
@@ -174,12 +179,20 @@ def test_mfs_flag_schedule_two_channels():
         print(f"stage flag {k}: gpu f {res.final_value:.8e} ({res.iterations} it), oracle "
               f"{ores.final_value:.8e} ({ores.iterations} it); first {traj[0]:.8e} vs {otraj[0]:.8e}")
         # (not monotone: at flag > 0 the clip zeroes alpha where I_nu_0 < threshold,
-        # in place, so the oracle's own values can rise within a stage)
-        assert abs(traj[0] - otraj[0]) <= 1e-3 * abs(otraj[0]), k
+        # in place, so the oracle's own values can rise within a stage).  Stage 0
+        # starts from the same image; later stages start from images that
+        # differ by the line search's amplification of rounding (the oracle
+        # with 1e-7 relative noise in f drifts by 4e-4 within 5 iterations).
+        tol = 1e-6 if k == 0 else 2e-2
+        assert abs(traj[0] - otraj[0]) <= tol * abs(otraj[0]), k
         assert abs(res.final_value - ores.final_value) <= 0.05 * abs(ores.final_value), k
     for k in range(4):
         moved, fixed = k % 2, 1 - k % 2
-        assert np.array_equal(images[k + 1][fixed], images[k][fixed]), k
+        # the image not optimised in a stage changes only where the clip of a
+        # flag > 0 evaluation zeroes alpha (I_nu_0 < threshold)
+        changed = images[k + 1][fixed] != images[k][fixed]
+        assert np.all(images[k + 1][fixed][changed] == 0), k
+        assert fixed == 1 or not changed.any(), k
         assert not np.array_equal(images[k + 1][moved], images[k][moved]), k
     ierr = np.linalg.norm(images[-1] - ref) / np.linalg.norm(ref)
     print(f"final image rel L2 {ierr:.3e}")

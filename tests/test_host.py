@@ -158,6 +158,39 @@ def test_generator_counts_and_determinism():
     assert 0.8e-3 < d.std() / np.sqrt(2) < 1.2e-3
 
 
+def test_generator_spectral_index_and_flagging():
+    """Multi-frequency generator: fluxes scale as (nu / nu0)^alpha (a point
+    source at the phase centre, no noise: every visibility equals its flux),
+    flagging packs the surviving rows in order, and the single-channel
+    workloads are unchanged by the new fields."""
+    fw = native.load_fw()
+    cfg = sim_config(3, npix=64, n_antennas=5, ha_start_hours=-0.05, ha_end_hours=0.05,
+                     n_gaussians=0, n_points=1, position_radius_frac=0.0, noise_sigma=1e-9,
+                     point_flux_min=0.5, point_flux_max=0.5, alpha_min=2.0, alpha_max=2.0)
+    n = fw.gvm_sim_count_visibilities(C.byref(cfg))
+    rows = {}
+    for f, frac in ((225e9, 0.0), (240e9, 0.0), (240e9, 0.3)):
+        cfg.freq_hz, cfg.flag_fraction = f, frac
+        uvw = np.zeros((n, 3))
+        vo = np.zeros((n, 2), np.float32)
+        w = np.zeros(n, np.float32)
+        nk = C.c_long()
+        assert fw.gvm_sim_generate_flagged(C.byref(cfg), uvw.ctypes.data, vo.ctypes.data,
+                                           w.ctypes.data, None, None, None, C.byref(nk)) == 0
+        rows[(f, frac)] = (uvw[:nk.value], vo[:nk.value])
+    # centre point: flux * PB(centre) = flux, times (nu/nu0)^alpha
+    assert np.allclose(rows[(225e9, 0.0)][1][:, 0], 0.5, rtol=1e-5)
+    assert np.allclose(rows[(240e9, 0.0)][1][:, 0], 0.5 * (240 / 225) ** 2, rtol=1e-5)
+    kept = rows[(240e9, 0.3)][0]
+    assert 0.5 * n < kept.shape[0] < 0.9 * n
+    full = rows[(240e9, 0.0)][0]
+    idx = [np.flatnonzero((full == r).all(axis=1))[0] for r in kept[:20]]
+    assert idx == sorted(idx)  # surviving rows keep their order
+    cfg.flag_fraction = 1.0
+    assert fw.gvm_sim_generate_flagged(C.byref(cfg), uvw.ctypes.data, vo.ctypes.data,
+                                       w.ctypes.data, None, None, None, C.byref(nk)) == 1
+
+
 def test_problem_setup_invariants():
     p = make_problem(0)
     assert p.nvis == 36000 and p.M == p.N == 256
