@@ -124,7 +124,8 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned int* hi,
     bf16x2_t h;
     h.x = (__bf16)x0;
     h.y = (__bf16)x1;
-    const unsigned int hb = __builtin_bit_cast(unsigned int, h);
+    unsigned int hb = __builtin_bit_cast(unsigned int, h);
+    asm("" : "+v"(hb));  // the hi parts below come from these packed bits
     bf16x2_t l;
     l.x = (__bf16)(x0 - __uint_as_float(hb << 16));
     l.y = (__bf16)(x1 - __uint_as_float(hb & 0xffff0000u));
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
       const float4 d = sm.rec_d[buf][kk];
       float sy, cy, sx, cx;
       sincos_turns(fract(s.z * yt.x) + fmaf(s.w, yt.x, fmaf(d.z, yt.y, d.y)), &sy, &cy);
-      sincos_turns(fract(s.x * xt.x) + fmaf(s.y, xt.x, d.z * xt.y), &sx, &cx);
+      sincos_turns(fmaf(d.z, xt.y, fmaf(s.y, xt.x, fract(s.x * xt.x))), &sx, &cx);
       pv[2 * q] = d.x * cy;
       pv[2 * q + 1] = -(d.x * sy);
       qv[2 * q] = cx;

@@ -104,7 +104,7 @@ struct gvm_ctx {
   int split_cap = 0;
   int nsplit_total = 0;
   int split_mode = -1;          // adjoint mode the table was built for (-1: stale)
-  int adjoint_mode = GVM_ADJOINT_F16X3;
+  int adjoint_mode = GVM_ADJOINT_BF16X3;
   std::vector<gvm::Slice> slices;
   bool have_residuals = false;
   // timing of the adjoint GEMM

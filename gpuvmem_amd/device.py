@@ -56,7 +56,7 @@ class DeviceBuffer:
 class Context:
     """A gvm_ctx with one slice per call to :meth:`add`."""
 
-    def __init__(self, problem: Problem, adjoint: str = "f16x3", add_data: bool = True):
+    def __init__(self, problem: Problem, adjoint: str = "bf16x3", add_data: bool = True):
         self.lib = native.load_hip()
         self.problem = problem
         g = problem.geometry()
