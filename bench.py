@@ -204,6 +204,51 @@ def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
     return {"target_objective": target, "forms": out}
 
 
+def precision_study(p, pixels: int = 64) -> dict:
+    """Each adjoint form against FP64: chi2 against the oracle's FP64
+    forward pass, the gradient against FP64 direct sums on a pixel sample
+    inside the mask, and the split forms against the FP32 and direct forms
+    over the whole image."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    from gpuvmem_amd.device import Context
+
+    img = bench_image(p)
+    orc = oracle.Oracle(p, separable=True)
+    ref_chi2, vr, _ = orc.chi2(img.astype(np.float64))
+    rng = np.random.default_rng(7)
+    inside = np.argwhere(p.noise < p.noise_cut)
+    pix = inside[rng.choice(len(inside), pixels, replace=False)]
+    gref = orc.dchi2_pixels(vr, pix)
+    out, full = {}, {}
+    for mode in ("direct", "f32", "f16x3", "bf16x3"):
+        ctx = Context(p, adjoint=mode)
+        d = ctx.image(img)
+        v = ctx.chi2(d)
+        res = ctx.zeros(2 * p.M * p.N)
+        ctx.dchi2(d, res)  # allocates the workspace
+        ctx.sync()
+        ctx.reset_timing()
+        res = ctx.zeros(2 * p.M * p.N)
+        ctx.dchi2(d, res)
+        ctx.sync()
+        ms, _ = ctx.adjoint_time_ms()
+        g = res.download(np.float32, (2, p.N, p.M))[0].astype(np.float64)
+        ctx.close()
+        gs = np.array([g[i, j] for i, j in pix])
+        full[mode] = g
+        out[mode] = {"chi2_rel_err": abs(v - ref_chi2) / abs(ref_chi2),
+                     "grad_rel_l2_vs_fp64_pixels": float(np.linalg.norm(gs - gref) / np.linalg.norm(gref)),
+                     "adjoint_ms": ms}
+        print(f"# precision {mode}: {out[mode]}", file=sys.stderr, flush=True)
+    for mode in ("f32", "f16x3", "bf16x3"):
+        for base in ("direct", "f32"):
+            if mode != base:
+                out[mode][f"grad_rel_l2_vs_{base}_image"] = float(
+                    np.linalg.norm(full[mode] - full[base]) / np.linalg.norm(full[base]))
+    return {"pixels": pixels, "forms": out}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,13 +256,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--adjoint", default="bf16x3", choices=list(PEAK_TFLOPS))
     ap.add_argument("--workload", type=int, default=1,
-                    help="1: 1024^2/1.76M vis, 0: 256^2/36k, 3: multi-frequency 512^2 I_nu_0 + "
-                         "alpha, 4 spw x 64 channels, ~61M vis")
+                    help="1: 1024^2/1.76M vis, 0: 256^2/36k, 2: 2048^2/4.35M vis, 3: "
+                         "multi-frequency 512^2 I_nu_0 + alpha, 4 spw x 64 channels, ~61M vis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--time-to-image", type=int, default=0, metavar="ITERS",
                     help="instead of evals/s: L-BFGS time-to-image of the split-precision "
                          "adjoint against the plain FP32 one at equal objective value")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="instead of evals/s: each adjoint form against FP64 (use with "
+                         "--workload 2: 2048^2, 4.35M vis)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size (for counter scaling)")
     args = ap.parse_args()
@@ -234,6 +282,11 @@ def main() -> int:
     print(f"# workload {args.workload}: {p.M}x{p.N}, {len(p.all_slices())} slices, "
           f"{p.nvis_total} visibilities "
           f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    if args.precision_study:
+        r = precision_study(p)
+        print(json.dumps({"metric": f"adjoint precision against FP64 ({p.M}^2, {p.nvis_total} vis)",
+                          "n_gpus": 1, "data": "synthetic", **r}), flush=True)
+        return 0
     if args.time_to_image > 0:
         r = time_to_image(p, args.time_to_image)
         f32, split = r["forms"]["f32"], r["forms"]["bf16x3"]

@@ -480,7 +480,7 @@ namespace {
 // slice.
 gvm_status build_splits(gvm_ctx* ctx, int mode, int ntiles, long n) {
   constexpr long kSplitVis = 3072;
-  constexpr long kMaxWorkspace = 16L << 30;
+  constexpr long kMaxWorkspace = 32L << 30;
   const int nsl = (int)ctx->slices.size();
   std::vector<int4> tab;
   std::vector<int> p0(nsl + 1, 0);

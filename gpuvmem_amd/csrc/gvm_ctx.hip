@@ -100,6 +100,21 @@ gvm_status finish_reductions_mode(gvm_ctx* ctx, int nslots, int nblocks, int is_
   return GVM_OK;
 }
 
+gvm_status finish_reductions_to(gvm_ctx* ctx, int nslots, int nblocks, double* out) {
+  hipLaunchKernelGGL(finish_kernel, dim3(nslots), dim3(kReduceThreads), 0, ctx->stream,
+                     ctx->partials, out, nblocks, 0);
+  GVM_CHECK_LAUNCH();
+  return GVM_OK;
+}
+
+gvm_status values_read(gvm_ctx* ctx, int n, double* out) {
+  GVM_HIP_TRY(hipMemcpyAsync(ctx->hvalues, ctx->dvalues, sizeof(double) * n,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int s = 0; s < n; s++) out[s] = ctx->hvalues[s];
+  return GVM_OK;
+}
+
 gvm_status finish_reductions(gvm_ctx* ctx, int nslots, int nblocks) {
   return finish_reductions_mode(ctx, nslots, nblocks, 0);
 }
@@ -221,6 +236,8 @@ extern "C" gvm_status gvm_destroy(gvm_ctx* ctx) {
   (void)hipFree(ctx->split_tab);
   (void)hipFree(ctx->slice_p0);
   if (ctx->hscalars) (void)hipHostFree(ctx->hscalars);
+  (void)hipFree(ctx->dvalues);
+  if (ctx->hvalues) (void)hipHostFree(ctx->hvalues);
   if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
   if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -303,6 +320,8 @@ static gvm_status create_impl(const gvm_geometry* g, const float* noise_host, gv
   if ((st = dev_alloc(&ctx->dscalars, kMaxSlots))) return st;
   if ((st = dev_alloc(&ctx->cmax, kMaxSlots))) return st;
   GVM_HIP_TRY(hipHostMalloc((void**)&ctx->hscalars, sizeof(double) * kMaxSlots));
+  if ((st = dev_alloc(&ctx->dvalues, kValueSlots))) return st;
+  GVM_HIP_TRY(hipHostMalloc((void**)&ctx->hvalues, sizeof(double) * kValueSlots));
   return GVM_OK;
 }
 
@@ -470,6 +489,22 @@ extern "C" gvm_status gvm_chi2(gvm_ctx* ctx, float* I_dev, int flag_opt, int nor
   GVM_REQUIRE(!ctx->slices.empty(), "gvm_chi2: no visibilities");
   GVM_REQUIRE(flag_opt >= 0, "gvm_chi2: negative flag_opt");
   return forward_chi2(ctx, I_dev, flag_opt, normalize, chi2_out);
+}
+
+extern "C" gvm_status gvm_chi2_enqueue(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize,
+                                       int slot) {
+  GVM_REQUIRE(ctx && I_dev, "gvm_chi2_enqueue: null argument");
+  GVM_REQUIRE(!ctx->slices.empty(), "gvm_chi2_enqueue: no visibilities");
+  GVM_REQUIRE(flag_opt >= 0, "gvm_chi2_enqueue: negative flag_opt");
+  GVM_REQUIRE(slot >= 0 && slot < kValueSlots, "gvm_chi2_enqueue: value slot out of range");
+  return forward_chi2_enqueue(ctx, I_dev, flag_opt, normalize, slot);
+}
+
+extern "C" gvm_status gvm_values_read(gvm_ctx* ctx, int n, double* out) {
+  GVM_REQUIRE(ctx && out, "gvm_values_read: null argument");
+  GVM_REQUIRE(n >= 1 && n <= kValueSlots, "gvm_values_read: slot count out of range");
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  return values_read(ctx, n, out);
 }
 
 extern "C" gvm_status gvm_dchi2(gvm_ctx* ctx, const float* I_dev, float* result_dev,

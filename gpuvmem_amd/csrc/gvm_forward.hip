@@ -293,7 +293,23 @@ __global__ __launch_bounds__(256) void degrid_chi2_kernel(
   }
 }
 
-gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize, double* out) {
+// 0.5 * the sum over slices of the per-slice sums (each divided by its
+// visibility count if normalize), in slice order, in FP64 (src/functions.cu:
+// 4377-4453): one thread, so the order is fixed.
+__global__ void chi2_combine_kernel(const double* __restrict__ sums,
+                                    const SliceDev* __restrict__ tab, int nsl, int normalize,
+                                    double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double total = 0.0;
+  for (int s = 0; s < nsl; s++) {
+    double r = sums[s];
+    if (normalize) r /= (double)tab[s].nvis;
+    total += r;
+  }
+  *out = 0.5 * total;
+}
+
+gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot) {
   const gvm_geometry& g = ctx->g;
   const long n = (long)g.M * g.N;
   const int nsl = (int)ctx->slices.size();
@@ -316,18 +332,17 @@ gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize, dou
   GVM_CHECK_LAUNCH();
   st = finish_reductions(ctx, nsl, nb);
   if (st) return st;
-  std::vector<double> sums(nsl);
-  st = read_scalars(ctx, nsl, sums.data());
-  if (st) return st;
-  double total = 0.0;
-  for (int si = 0; si < nsl; si++) {
-    double r = sums[si];
-    if (normalize) r /= (double)ctx->slices[si].nvis;
-    total += r;
-  }
+  hipLaunchKernelGGL(chi2_combine_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->dscalars,
+                     ctx->slice_tab, nsl, normalize, ctx->dvalues + slot);
+  GVM_CHECK_LAUNCH();
   ctx->have_residuals = true;
-  *out = 0.5 * total;
   return GVM_OK;
+}
+
+gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize, double* out) {
+  gvm_status st = forward_chi2_enqueue(ctx, I, flag_opt, normalize, 0);
+  if (st) return st;
+  return values_read(ctx, 1, out);
 }
 
 }  // namespace gvm

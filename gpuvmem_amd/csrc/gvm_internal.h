@@ -71,6 +71,7 @@ struct SliceDev {
 constexpr int kReduceBlocks = 1024;
 constexpr int kReduceThreads = 256;
 constexpr int kMaxSlots = 4096;  // slices per context (and reduction slots)
+constexpr int kValueSlots = 16;  // term values enqueued before one read-back
 
 }  // namespace gvm
 
@@ -91,6 +92,8 @@ struct gvm_ctx {
   double* partials = nullptr;   // kMaxSlots * kReduceBlocks doubles
   double* dscalars = nullptr;   // kMaxSlots doubles (device)
   double* hscalars = nullptr;   // kMaxSlots doubles (pinned host)
+  double* dvalues = nullptr;    // kValueSlots term values (device), read back together
+  double* hvalues = nullptr;    // kValueSlots doubles (pinned host)
   unsigned int* cmax = nullptr; // per-slice max |w Vr| (float bits), device
   // batched slices: device descriptor table, uv grids (one per slice)
   gvm::SliceDev* slice_tab = nullptr;  // kMaxSlots entries
@@ -124,6 +127,12 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt,
 // dscalars[s] (device), in a fixed order.
 gvm_status finish_reductions(gvm_ctx* ctx, int nslots, int nblocks);
 gvm_status finish_reductions_mode(gvm_ctx* ctx, int nslots, int nblocks, int is_max);
+// The same, writing the nslots results to `out` (device) instead of dscalars.
+gvm_status finish_reductions_to(gvm_ctx* ctx, int nslots, int nblocks, double* out);
+// Enqueues the chi2 evaluation; 0.5 * sum over slices lands in dvalues[slot].
+gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot);
+// Copies dvalues[0, n) to the host (one synchronisation).
+gvm_status values_read(gvm_ctx* ctx, int n, double* out);
 gvm_status read_scalars(gvm_ctx* ctx, int nslots, double* out);
 int reduce_grid(long n);
 

@@ -89,10 +89,11 @@ __global__ __launch_bounds__(256) void prior_grad_kernel(int kind, const float* 
   }
 }
 
-static gvm_status prior_value(gvm_ctx* ctx, int kind, const float* I, int index, float G,
-                              float eta, double* out) {
-  GVM_REQUIRE(ctx && I && out, "prior value: null argument");
+static gvm_status prior_value_enqueue(gvm_ctx* ctx, int kind, const float* I, int index,
+                                      float G, float eta, int slot) {
+  GVM_REQUIRE(ctx && I, "prior value: null argument");
   GVM_REQUIRE(index >= 0 && index < ctx->g.image_count, "prior value: bad image index");
+  GVM_REQUIRE(slot >= 0 && slot < kValueSlots, "prior value: value slot out of range");
   GVM_HIP_TRY(hipSetDevice(ctx->device));
   const int N = ctx->g.N;
   const long n = (long)N * N;
@@ -100,9 +101,15 @@ static gvm_status prior_value(gvm_ctx* ctx, int kind, const float* I, int index,
   hipLaunchKernelGGL(prior_value_kernel, dim3(nb), dim3(256), 0, ctx->stream, kind, I + n * index,
                      ctx->noise, ctx->g.noise_cut, N, G, eta, ctx->partials);
   GVM_CHECK_LAUNCH();
-  gvm_status st = finish_reductions(ctx, 1, nb);
+  return finish_reductions_to(ctx, 1, nb, ctx->dvalues + slot);
+}
+
+static gvm_status prior_value(gvm_ctx* ctx, int kind, const float* I, int index, float G,
+                              float eta, double* out) {
+  GVM_REQUIRE(out, "prior value: null argument");
+  gvm_status st = prior_value_enqueue(ctx, kind, I, index, G, eta, 0);
   if (st) return st;
-  return read_scalars(ctx, 1, out);
+  return values_read(ctx, 1, out);
 }
 
 static gvm_status prior_grad(gvm_ctx* ctx, int kind, const float* I, float* dgi, int index,
@@ -259,6 +266,20 @@ extern "C" gvm_status gvm_entropy_grad(gvm_ctx* ctx, const float* I, float* dgi,
                                        float prior, float eta, float lambda) {
   return prior_grad(ctx, PRIOR_ENTROPY, I, dgi, index, prior, eta, lambda);
 }
+extern "C" gvm_status gvm_entropy_value_enqueue(gvm_ctx* ctx, const float* I, int index,
+                                                float prior, float eta, int slot) {
+  return prior_value_enqueue(ctx, PRIOR_ENTROPY, I, index, prior, eta, slot);
+}
+
+extern "C" gvm_status gvm_tsv_value_enqueue(gvm_ctx* ctx, const float* I, int index, int slot) {
+  return prior_value_enqueue(ctx, PRIOR_TSV, I, index, 0.f, 0.f, slot);
+}
+
+extern "C" gvm_status gvm_laplacian_value_enqueue(gvm_ctx* ctx, const float* I, int index,
+                                                  int slot) {
+  return prior_value_enqueue(ctx, PRIOR_LAPLACIAN, I, index, 0.f, 0.f, slot);
+}
+
 extern "C" gvm_status gvm_tsv_value(gvm_ctx* ctx, const float* I, int index, double* out) {
   return prior_value(ctx, PRIOR_TSV, I, index, 0.f, 0.f, out);
 }

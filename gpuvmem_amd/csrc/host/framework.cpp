@@ -86,6 +86,20 @@ void ObjectiveFunction::addFi(Fi* fi) {
 double ObjectiveFunction::calcFunctionD(float* p) {
   double value = 0.0;
   n_fcalls++;
+  bool async = !fis.empty() && fis.size() <= GVM_VALUE_SLOTS;
+  for (Fi* fi : fis) async = async && fi->hasAsyncValue();
+  if (async) {
+    // the same evaluations in the same stream order, one synchronisation
+    std::vector<char> queued(fis.size());
+    for (size_t k = 0; k < fis.size(); k++) queued[k] = fis[k]->enqueueValue(p, (int)k) ? 1 : 0;
+    std::vector<double> raw(fis.size(), 0.0);
+    check(gvm_values_read(gvm::ctx(), (int)fis.size(), raw.data()), "gvm_values_read");
+    for (size_t k = 0; k < fis.size(); k++) {
+      value += fis[k]->finishValue(queued[k] ? raw[k] : 0.0);
+      fi_values[k] = fis[k]->get_fivalue();
+    }
+    return value;
+  }
   for (size_t k = 0; k < fis.size(); k++) {
     value += fis[k]->calcFiD(p);
     fi_values[k] = fis[k]->get_fivalue();
@@ -166,6 +180,12 @@ double Chi2::calcFiD(float* p) {
 
 float Chi2::calcFi(float* p) { return (float)calcFiD(p); }
 
+bool Chi2::enqueueValue(float* p, int slot) {
+  check(gvm_chi2_enqueue(gvm::ctx(), p, globals().flag_opt, normalize ? 1 : 0, slot),
+        "gvm_chi2_enqueue");
+  return true;
+}
+
 void Chi2::calcGi(float* p, float* xi) {
   check(gvm_dchi2(gvm::ctx(), p, result_dchi2, globals().flag_opt, normalize ? 1 : 0),
         "gvm_dchi2");
@@ -209,6 +229,12 @@ double Entropy::calcFiD(float* p) {
   return (double)penalization_factor * v;
 }
 float Entropy::calcFi(float* p) { return (float)calcFiD(p); }
+bool Entropy::enqueueValue(float* p, int slot) {
+  if (!prior_active(iteration, penalization_factor)) return false;
+  check(gvm_entropy_value_enqueue(gvm::ctx(), p, imageIndex, prior_value, eta, slot),
+        "gvm_entropy_value_enqueue");
+  return true;
+}
 
 void Entropy::calcGi(float* p, float* xi) {
   if (prior_active(iteration, penalization_factor) && globals().flag_opt % 2 == imageIndex)
@@ -232,6 +258,11 @@ double TotalSquaredVariationP::calcFiD(float* p) {
   return (double)penalization_factor * v;
 }
 float TotalSquaredVariationP::calcFi(float* p) { return (float)calcFiD(p); }
+bool TotalSquaredVariationP::enqueueValue(float* p, int slot) {
+  if (!prior_active(iteration, penalization_factor)) return false;
+  check(gvm_tsv_value_enqueue(gvm::ctx(), p, imageIndex, slot), "gvm_tsv_value_enqueue");
+  return true;
+}
 void TotalSquaredVariationP::calcGi(float* p, float* xi) {
   if (prior_active(iteration, penalization_factor) && globals().flag_opt % 2 == imageIndex)
     check(gvm_tsv_grad(gvm::ctx(), p, device_DS, imageIndex, penalization_factor), "gvm_tsv_grad");
@@ -252,6 +283,12 @@ double Laplacian::calcFiD(float* p) {
   return (double)penalization_factor * v;
 }
 float Laplacian::calcFi(float* p) { return (float)calcFiD(p); }
+bool Laplacian::enqueueValue(float* p, int slot) {
+  if (!prior_active(iteration, penalization_factor)) return false;
+  check(gvm_laplacian_value_enqueue(gvm::ctx(), p, imageIndex, slot),
+        "gvm_laplacian_value_enqueue");
+  return true;
+}
 void Laplacian::calcGi(float* p, float* xi) {
   if (prior_active(iteration, penalization_factor) && globals().flag_opt % 2 == imageIndex)
     check(gvm_laplacian_grad(gvm::ctx(), p, device_DS, imageIndex, penalization_factor),

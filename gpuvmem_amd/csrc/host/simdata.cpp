@@ -260,7 +260,30 @@ extern "C" void gvm_sim_default_config(int which, gvm_sim_config* c) {
   c->point_flux_max = 1.0;
   c->alpha_min = 0.0;
   c->alpha_max = 0.0;
-  if (which == 3) {
+  if (which == 2) {
+    // 2048^2 precision workload: 64 antennas uniform in a 5 km disk, 6 h at
+    // 10 s (2016 x 2160 = 4,354,560 rows), 60 GHz so the longest baselines
+    // reach about 1e6 wavelengths; 200 Gaussian components (power-law fluxes).
+    c->npix = 2048;
+    c->n_antennas = 64;
+    c->layout = GVM_SIM_LAYOUT_UNIFORM;
+    c->array_diameter_m = 5000.0;
+    c->ha_start_hours = -3.0;
+    c->ha_end_hours = 3.0;
+    c->dump_seconds = 10.0;
+    c->freq_hz = 60e9;
+    c->n_gaussians = 200;
+    c->n_points = 0;
+    c->sigma_min_pix = 1.0;
+    c->sigma_max_pix = 8.0;
+    c->flux_law = GVM_SIM_FLUX_POWERLAW;
+    c->flux_index = -1.5;
+    c->flux_min = 1e-3;
+    c->flux_max = 1.0;
+    c->seed_layout = 20;
+    c->seed_sky = 21;
+    c->seed_noise = 22;
+  } else if (which == 3) {
     // One channel of the 512^2 multi-frequency workload: 35 antennas
     // Gaussian-radial in a 2 km disk, 504 dumps of 10 s (595 x 504 = 299,880
     // rows); 30 Gaussian components (power-law fluxes) + 5 points with

@@ -97,6 +97,11 @@ HIP_API = {
     "gvm_get_geometry": [_P, C.POINTER(Geometry)],
     "gvm_set_adjoint": [_P, _I],
     "gvm_chi2": [_P, _P, _I, _I, _DP],
+    "gvm_chi2_enqueue": [_P, _P, _I, _I, _I],
+    "gvm_entropy_value_enqueue": [_P, _P, _I, _F, _F, _I],
+    "gvm_tsv_value_enqueue": [_P, _P, _I, _I],
+    "gvm_laplacian_value_enqueue": [_P, _P, _I, _I],
+    "gvm_values_read": [_P, _I, _DP],
     "gvm_dchi2": [_P, _P, _P, _I, _I],
     "gvm_get_residuals": [_P, _I, _P],
     "gvm_entropy_value": [_P, _P, _I, _F, _F, _DP],

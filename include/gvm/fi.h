@@ -21,6 +21,17 @@ class Fi {
   virtual void restartDGi() = 0;
   virtual void addToDphi(float* device_dphi) = 0;
   virtual double calcFiD(float* p) { return (double)calcFi(p); }
+  // Asynchronous value (not in the reference): when every term of an
+  // ObjectiveFunction has one, it enqueues all term values and waits once.
+  // enqueueValue returns false when the value is zero without evaluation
+  // (an inactive prior); finishValue stores the raw value like calcFi and
+  // returns the weighted one.
+  virtual bool hasAsyncValue() const { return false; }
+  virtual bool enqueueValue(float* p, int slot) { return false; }
+  double finishValue(double raw) {
+    set_fivalue(raw);
+    return (double)penalization_factor * raw;
+  }
   virtual void setPrior(float prior) {}
   virtual void setPrior(float* prior) {}
   virtual float getEta() { return 0.0f; }

@@ -53,6 +53,7 @@ typedef struct {
 } gvm_sim_component;
 
 /* which: 0 = 256^2 / 36,000 vis workload, 1 = 1024^2 / 1,764,000 vis,
+ * 2 = 2048^2 / 4,354,560 vis at 60 GHz (baselines to about 1e6 wavelengths),
  * 3 = one channel of the 512^2 multi-frequency workload (35 antennas,
  * 595 baselines x 504 dumps = 299,880 rows per channel, nu0 = 225 GHz,
  * spectral indices U(-1, 3); the caller sets freq_hz, flag_fraction and the

@@ -12,6 +12,8 @@ class Chi2 : public Fi {
   ~Chi2() override;
   float calcFi(float* p) override;
   double calcFiD(float* p) override;
+  bool hasAsyncValue() const override { return true; }
+  bool enqueueValue(float* p, int slot) override;
   void calcGi(float* p, float* xi) override;
   void restartDGi() override;
   void addToDphi(float* device_dphi) override;
@@ -31,6 +33,8 @@ class Entropy : public Fi {
   Entropy(float prior_value, float eta);
   float calcFi(float* p) override;
   double calcFiD(float* p) override;
+  bool hasAsyncValue() const override { return true; }
+  bool enqueueValue(float* p, int slot) override;
   void calcGi(float* p, float* xi) override;
   void restartDGi() override;
   void addToDphi(float* device_dphi) override;
@@ -49,6 +53,8 @@ class TotalSquaredVariationP : public Fi {
   TotalSquaredVariationP();
   float calcFi(float* p) override;
   double calcFiD(float* p) override;
+  bool hasAsyncValue() const override { return true; }
+  bool enqueueValue(float* p, int slot) override;
   void calcGi(float* p, float* xi) override;
   void restartDGi() override;
   void addToDphi(float* device_dphi) override;
@@ -59,6 +65,8 @@ class Laplacian : public Fi {
   Laplacian();
   float calcFi(float* p) override;
   double calcFiD(float* p) override;
+  bool hasAsyncValue() const override { return true; }
+  bool enqueueValue(float* p, int slot) override;
   void calcGi(float* p, float* xi) override;
   void restartDGi() override;
   void addToDphi(float* device_dphi) override;

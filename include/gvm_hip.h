@@ -114,6 +114,21 @@ gvm_status gvm_set_adjoint(gvm_ctx* ctx, int mode);
 gvm_status gvm_chi2(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize,
                     double* chi2_out);
 
+/* Asynchronous value forms, with no counterpart in the reference: they
+ * enqueue an evaluation on the context's stream and leave its FP64 value in
+ * device value slot `slot` (0..15) instead of returning it, so a caller can
+ * evaluate several terms of an objective with one synchronisation
+ * (gvm_values_read).  Results are identical to the synchronous forms.  A slot
+ * keeps its value until it is written again. */
+#define GVM_VALUE_SLOTS 16
+gvm_status gvm_chi2_enqueue(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize, int slot);
+gvm_status gvm_entropy_value_enqueue(gvm_ctx* ctx, const float* I_dev, int index, float prior,
+                                     float eta, int slot);
+gvm_status gvm_tsv_value_enqueue(gvm_ctx* ctx, const float* I_dev, int index, int slot);
+gvm_status gvm_laplacian_value_enqueue(gvm_ctx* ctx, const float* I_dev, int index, int slot);
+/* Waits for the stream and copies value slots [0, n) to out. */
+gvm_status gvm_values_read(gvm_ctx* ctx, int n, double* out);
+
 /* Replaces dchi2() (src/functions.cu:4456-4558) as called by Chi2::calcGi
  * (src/chi2.cu:51-53): per slice the DChi2 adjoint (:3698-3791) at the
  * residuals of the last gvm_chi2, then DChi2_total_I_nu_0 (:4000-4024) when

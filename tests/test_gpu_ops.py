@@ -156,6 +156,28 @@ def test_lbfgs_vector_ops(setup):
     assert np.all(y.download() == 0)
 
 
+def test_enqueued_values_match_synchronous(setup):
+    """The asynchronous value forms leave exactly the synchronous results in
+    the chosen value slots, read back together with one call."""
+    p, ctx, _ = setup
+    lib = ctx.lib
+    img = _img(p, 6)
+    d = ctx.image(img)
+    chi2 = ctx.chi2(d)
+    ent = ctx.prior_value("entropy", d, index=0, prior=0.001, eta=-1.0)
+    tsv = ctx.prior_value("tsv", d, index=1)
+    lap = ctx.prior_value("laplacian", d, index=0)
+    native.check(lib.gvm_chi2_enqueue(ctx.ctx, d.ptr, 0, 0, 3), "chi2 enqueue")
+    native.check(lib.gvm_entropy_value_enqueue(ctx.ctx, d.ptr, 0, 0.001, -1.0, 5), "entropy")
+    native.check(lib.gvm_tsv_value_enqueue(ctx.ctx, d.ptr, 1, 1), "tsv")
+    native.check(lib.gvm_laplacian_value_enqueue(ctx.ctx, d.ptr, 0, 15), "laplacian")
+    out = (C.c_double * 16)()
+    native.check(lib.gvm_values_read(ctx.ctx, 16, out), "read")
+    assert (out[3], out[5], out[1], out[15]) == (chi2, ent, tsv, lap)
+    assert lib.gvm_tsv_value_enqueue(ctx.ctx, d.ptr, 1, 16) == 1  # slot out of range
+    assert lib.gvm_values_read(ctx.ctx, 17, out) == 1
+
+
 def test_dchi2_requires_chi2_first():
     p = small_problem(npix=64, n_antennas=5)
     ctx = Context(p)
