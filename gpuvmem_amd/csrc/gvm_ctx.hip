@@ -119,6 +119,73 @@ gvm_status finish_reductions(gvm_ctx* ctx, int nslots, int nblocks) {
   return finish_reductions_mode(ctx, nslots, nblocks, 0);
 }
 
+// Frees the device arrays of a slice.
+void free_slice(Slice& s) {
+  (void)hipFree(s.cell);
+  (void)hipFree(s.frac);
+  (void)hipFree(s.vo);
+  (void)hipFree(s.weight);
+  (void)hipFree(s.phs);
+  (void)hipFree(s.wcoord);
+  (void)hipFree(s.pb);
+  (void)hipFree(s.vr);
+  (void)hipFree(s.cpol);
+  s = Slice{};
+}
+
+// Makes s (device arrays filled, vr / cpol allocated) the next slice of ctx:
+// clears the per-evaluation arrays, grows the per-slice uv grids, writes the
+// slice's device descriptor.  On failure the slice's arrays are freed.
+gvm_status register_slice(gvm_ctx* ctx, Slice& s) {
+  const gvm_geometry& g = ctx->g;
+  auto fail = [&](gvm_status st) {
+    free_slice(s);
+    return st;
+  };
+  if (hipMemset(s.vr, 0, sizeof(float2) * s.nvis) != hipSuccess ||
+      hipMemset(s.cpol, 0, sizeof(float2) * s.nvis) != hipSuccess) {
+    set_error("register_slice: hipMemset failed");
+    return fail(GVM_ERR_HIP);
+  }
+  // one uv grid per slice for the batched forward pass
+  const size_t need = ctx->slices.size() + 1;
+  if (need > ctx->grid_slices) {
+    const size_t cap = std::max(need, 2 * ctx->grid_slices);
+    float2* grid = nullptr;
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMalloc((void**)&grid, sizeof(float2) * (size_t)g.M * g.N * cap) != hipSuccess) {
+      set_error("gvm_add_visibilities: out of device memory for the uv grids");
+      return fail(GVM_ERR_OOM);
+    }
+    (void)hipFree(ctx->grid);
+    ctx->grid = grid;
+    ctx->grid_slices = cap;
+  }
+  SliceDev d;
+  d.cell = s.cell;
+  d.frac = s.frac;
+  d.vo = s.vo;
+  d.weight = s.weight;
+  d.vr = s.vr;
+  d.cpol = s.cpol;
+  d.phs = s.phs;
+  d.wcoord = s.wcoord;
+  d.pb = s.pb;
+  d.nvis = s.nvis;
+  d.nudiv = s.freq / g.nu_0;
+  d.pad = 0.f;
+  if (hipMemcpy(ctx->slice_tab + ctx->slices.size(), &d, sizeof(d), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
+    set_error("register_slice: descriptor upload failed");
+    return fail(GVM_ERR_HIP);
+  }
+  ctx->slices.push_back(s);
+  ctx->split_mode = -1;
+  ctx->have_residuals = false;
+  return GVM_OK;
+}
+
 gvm_status read_scalars(gvm_ctx* ctx, int nslots, double* out) {
   GVM_HIP_TRY(hipMemcpyAsync(ctx->hscalars, ctx->dscalars, sizeof(double) * nslots,
                              hipMemcpyDeviceToHost, ctx->stream));
@@ -203,18 +270,6 @@ extern "C" gvm_status gvm_primary_beam(int beam_kind, double antenna_diameter,
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
-static void free_slice(Slice& s) {
-  (void)hipFree(s.cell);
-  (void)hipFree(s.frac);
-  (void)hipFree(s.vo);
-  (void)hipFree(s.weight);
-  (void)hipFree(s.phs);
-  (void)hipFree(s.wcoord);
-  (void)hipFree(s.pb);
-  (void)hipFree(s.vr);
-  (void)hipFree(s.cpol);
-  s = Slice{};
-}
 
 extern "C" gvm_status gvm_destroy(gvm_ctx* ctx) {
   if (!ctx) return GVM_OK;
@@ -434,41 +489,8 @@ extern "C" gvm_status gvm_add_visibilities(gvm_ctx* ctx, float freq, long nvis,
     free_slice(s);
     return st;
   }
-  GVM_HIP_TRY(hipMemset(s.vr, 0, sizeof(float2) * nvis));
-  GVM_HIP_TRY(hipMemset(s.cpol, 0, sizeof(float2) * nvis));
-  // one uv grid per slice for the batched forward pass
-  const size_t need = ctx->slices.size() + 1;
-  if (need > ctx->grid_slices) {
-    const size_t cap = std::max(need, 2 * ctx->grid_slices);
-    float2* grid = nullptr;
-    GVM_HIP_TRY(hipDeviceSynchronize());
-    if (hipMalloc((void**)&grid, sizeof(float2) * (size_t)g.M * g.N * cap) != hipSuccess) {
-      free_slice(s);
-      set_error("gvm_add_visibilities: out of device memory for the uv grids");
-      return GVM_ERR_OOM;
-    }
-    (void)hipFree(ctx->grid);
-    ctx->grid = grid;
-    ctx->grid_slices = cap;
-  }
-  SliceDev d;
-  d.cell = s.cell;
-  d.frac = s.frac;
-  d.vo = s.vo;
-  d.weight = s.weight;
-  d.vr = s.vr;
-  d.cpol = s.cpol;
-  d.phs = s.phs;
-  d.wcoord = s.wcoord;
-  d.pb = s.pb;
-  d.nvis = s.nvis;
-  d.nudiv = freq / g.nu_0;
-  d.pad = 0.f;
-  GVM_HIP_TRY(hipMemcpy(ctx->slice_tab + ctx->slices.size(), &d, sizeof(d), hipMemcpyHostToDevice));
-  GVM_HIP_TRY(hipDeviceSynchronize());
-  ctx->slices.push_back(s);
-  ctx->split_mode = -1;
-  ctx->have_residuals = false;
+  st = register_slice(ctx, s);
+  if (st) return st;
   if (slice_out) *slice_out = (int)ctx->slices.size() - 1;
   return GVM_OK;
 }

@@ -133,6 +133,12 @@ gvm_status finish_reductions_to(gvm_ctx* ctx, int nslots, int nblocks, double* o
 gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot);
 // Copies dvalues[0, n) to the host (one synchronisation).
 gvm_status values_read(gvm_ctx* ctx, int n, double* out);
+// Slices: frees one slice's device arrays; makes a filled slice the next
+// slice of ctx (grids, descriptor), freeing it on failure.
+void free_slice(Slice& s);
+gvm_status register_slice(gvm_ctx* ctx, Slice& s);
+// Primary beam image on the device (FP64 formula, stored as float).
+gvm_status primary_beam_dev(const gvm_geometry& g, double freq, float* out, hipStream_t stream);
 gvm_status read_scalars(gvm_ctx* ctx, int nslots, double* out);
 int reduce_grid(long n);
 

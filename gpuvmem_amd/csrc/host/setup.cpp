@@ -50,22 +50,8 @@ extern "C" int gvm_setup_beam_noise(int nch, const float* freq, const long* nvis
     }
   }
   if (!(sw > 0.0)) return GVM_ERR_ARG;  // "The sum of the visibility weights cannot be zero"
-  suu /= sw;
-  svv /= sw;
-  suv /= sw;
-  const double d = suu - svv, s = suu + svv;
-  const double root = std::sqrt(d * d + 4.0 * suv * suv);
-  const double bmaj = 1.0 / std::sqrt(2.0) / kPi / std::sqrt(s - root);
-  const double bmin = 1.0 / std::sqrt(2.0) / kPi / std::sqrt(s + root);
-  *bmaj_deg = bmaj / kRad;
-  *bmin_deg = bmin / kRad;
-  *bpa_deg = -0.5 * std::atan2(2.0 * suv, d) / kRad;
-  *sum_weights = sw;
-  *vis_noise = 0.5 * std::sqrt(1.0 / sw);
-  const double bmaj_pix = *bmaj_deg / std::fabs(deltax_deg);
-  const double bmin_pix = *bmin_deg / std::fabs(deltax_deg);
-  *noise_jypix = *vis_noise / (kPi * bmaj_pix * bmin_pix / (4.0 * std::log(2.0)));
-  return GVM_OK;
+  return gvm_setup_beam_from_moments(suu, svv, suv, sw, deltax_deg, bmaj_deg, bmin_deg, bpa_deg,
+                                     sum_weights, vis_noise, noise_jypix);
 }
 
 extern "C" double gvm_setup_pb_cutoff(double pb_factor, double max_wavelength, double diameter) {

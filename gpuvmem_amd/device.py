@@ -80,6 +80,16 @@ class Context:
                      "gvm_add_visibilities")
         return s.value
 
+    def add_dev(self, freq: float, uvw_m: "DeviceBuffer", vo: "DeviceBuffer", weight: "DeviceBuffer",
+                nvis: int) -> int:
+        """A slice from device arrays in the reference's raw form (uvw in
+        metres before the Hermitian flip), set up on the device."""
+        s = C.c_int()
+        native.check(self.lib.gvm_add_visibilities_dev(self.ctx, freq, nvis, uvw_m.ptr, vo.ptr,
+                                                       weight.ptr, C.byref(s)),
+                     "gvm_add_visibilities_dev")
+        return s.value
+
     def set_adjoint(self, mode: str) -> None:
         native.check(self.lib.gvm_set_adjoint(self.ctx, native.ADJOINT_MODES[mode]), "gvm_set_adjoint")
 

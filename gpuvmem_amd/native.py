@@ -98,6 +98,11 @@ HIP_API = {
     "gvm_set_adjoint": [_P, _I],
     "gvm_chi2": [_P, _P, _I, _I, _DP],
     "gvm_chi2_enqueue": [_P, _P, _I, _I, _I],
+    "gvm_add_visibilities_dev": [_P, _F, _L, _P, _P, _P, _IP],
+    "gvm_setup_beam_noise_dev": [_I, _P, _P, _P, _P, _D, _DP, _DP, _DP, _DP, _DP, _DP],
+    "gvm_setup_beam_from_moments": [_D, _D, _D, _D, _D, _DP, _DP, _DP, _DP, _DP, _DP],
+    "gvm_setup_noise_image_dev": [_I, _D, _D, _D, _D, _I, _I, _D, _D, _F, _F, _D, _D, _P,
+                                  C.POINTER(C.c_float), C.POINTER(C.c_float)],
     "gvm_entropy_value_enqueue": [_P, _P, _I, _F, _F, _I],
     "gvm_tsv_value_enqueue": [_P, _P, _I, _I],
     "gvm_laplacian_value_enqueue": [_P, _P, _I, _I],

@@ -82,6 +82,40 @@ typedef struct {
  * uv grid and work buffers, creates the context's stream. */
 gvm_status gvm_create(const gvm_geometry* geom, const float* noise_host,
                       gvm_ctx** out);
+/* Set-up on the device, for visibilities already in device memory (no
+ * counterpart entry point in the reference, whose set-up kernels these are):
+ * hermitianSymmetry (src/functions.cu:2256-2273: rows with u > 0 get u, v and
+ * Im Vo negated; metres to wavelengths with the float wavelength), then what
+ * gvm_add_visibilities does, in FP64 on the device.  uvw_m_dev: 3*nvis
+ * doubles in metres; vo_dev: 2*nvis floats; weight_dev: nvis floats (all
+ * device, left unchanged).  The slice equals the one gvm_add_visibilities
+ * builds from the same rows after gvm_setup_hermitian_lambda. */
+gvm_status gvm_add_visibilities_dev(gvm_ctx* ctx, float freq, long nvis,
+                                    const double* uvw_m_dev, const float* vo_dev,
+                                    const float* weight_dev, int* slice_out);
+/* calculateNoiseAndBeam (src/functions.cu:1700-1814) with the weighted uv
+ * moments summed on the device (device uvw in metres and weights per
+ * channel); outputs as gvm_setup_beam_noise. */
+int gvm_setup_beam_noise_dev(int nch, const float* freq, const long* nvis,
+                             const double* const* uvw_m_dev, const float* const* weight_dev,
+                             double deltax_deg, double* bmaj_deg, double* bmin_deg,
+                             double* bpa_deg, double* sum_weights, double* vis_noise,
+                             double* noise_jypix);
+/* The clean beam and the noise estimates from the weighted moments
+ * sum w u^2, sum w v^2, sum w u v, sum w (src/functions.cu:1760-1814). */
+int gvm_setup_beam_from_moments(double suu, double svv, double suv, double sw,
+                                double deltax_deg, double* bmaj_deg, double* bmin_deg,
+                                double* bpa_deg, double* sum_weights, double* vis_noise,
+                                double* noise_jypix);
+/* The noise image (src/mfs.cu:849-916, noise_image src/functions.cu:2382-2422)
+ * computed on the device and copied to noise_out (host, float[M*N]), with
+ * fg_scale = its minimum and noise_cut = noise_cut_factor * fg_scale. */
+int gvm_setup_noise_image_dev(int beam_kind, double diameter, double pb_factor,
+                              double pb_cutoff, double nu_0, int M, int N, double deltax_deg,
+                              double deltay_deg, float x0_ref, float y0_ref, double noise_jypix,
+                              double noise_cut_factor, float* noise_out, float* fg_scale,
+                              float* noise_cut);
+
 /* Replaces MFS::unSetDevice (src/mfs.cu:1157-1253). */
 gvm_status gvm_destroy(gvm_ctx* ctx);
 
