@@ -23,7 +23,8 @@ LIB = PKG / "lib"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["gvm_ctx.hip", "gvm_forward.hip", "gvm_adjoint.hip", "gvm_ops.hip", "gvm_setup.hip"]
-HOST_SOURCES = ["framework.cpp", "optimizers.cpp", "setup.cpp", "fw_capi.cpp", "simdata.cpp"]
+HOST_SOURCES = ["framework.cpp", "optimizers.cpp", "setup.cpp", "fw_capi.cpp", "simdata.cpp",
+                "fitsio.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")

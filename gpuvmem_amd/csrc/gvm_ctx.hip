@@ -495,6 +495,38 @@ extern "C" gvm_status gvm_add_visibilities(gvm_ctx* ctx, float freq, long nvis,
   return GVM_OK;
 }
 
+extern "C" gvm_status gvm_get_visibilities_out(gvm_ctx* ctx, int slice, const double* uvw_obs,
+                                               float* model_host, float* residual_host) {
+  GVM_REQUIRE(ctx && (model_host || residual_host), "gvm_get_visibilities_out: null argument");
+  GVM_REQUIRE(slice >= 0 && slice < (int)ctx->slices.size(),
+              "gvm_get_visibilities_out: bad slice");
+  if (!ctx->have_residuals) {
+    set_error("gvm_get_visibilities_out: no residuals (call gvm_chi2 first)");
+    return GVM_ERR_STATE;
+  }
+  const Slice& s = ctx->slices[slice];
+  std::vector<float2> vr(s.nvis), vo(s.nvis);
+  GVM_HIP_TRY(hipMemcpyAsync(vr.data(), s.vr, sizeof(float2) * s.nvis, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  GVM_HIP_TRY(hipMemcpyAsync(vo.data(), s.vo, sizeof(float2) * s.nvis, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (long i = 0; i < s.nvis; i++) {
+    // Vm = Vo - Vr in the flipped frame; conjugate back where the observed u > 0
+    const bool flipped = uvw_obs && uvw_obs[3 * i] > 0.0;
+    const float mr = vo[i].x - vr[i].x, mi = vo[i].y - vr[i].y;
+    if (model_host) {
+      model_host[2 * i] = mr;
+      model_host[2 * i + 1] = flipped ? -mi : mi;
+    }
+    if (residual_host) {
+      residual_host[2 * i] = vr[i].x;
+      residual_host[2 * i + 1] = flipped ? -vr[i].y : vr[i].y;
+    }
+  }
+  return GVM_OK;
+}
+
 extern "C" gvm_status gvm_get_residuals(gvm_ctx* ctx, int slice, float* vr_host) {
   GVM_REQUIRE(ctx && vr_host, "gvm_get_residuals: null argument");
   GVM_REQUIRE(slice >= 0 && slice < (int)ctx->slices.size(), "gvm_get_residuals: bad slice");

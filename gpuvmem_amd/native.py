@@ -62,6 +62,14 @@ class SimComponent(C.Structure):
                 ("sigma_pix", C.c_double), ("alpha", C.c_double)]
 
 
+class FitsHeader(C.Structure):
+    _fields_ = [("crpix1", C.c_double), ("crpix2", C.c_double), ("crval1", C.c_double),
+                ("crval2", C.c_double), ("cdelt1", C.c_double), ("cdelt2", C.c_double),
+                ("crval3", C.c_double), ("cdelt3", C.c_double), ("equinox", C.c_float),
+                ("radesys", C.c_char_p), ("bunit", C.c_char_p), ("object", C.c_char_p),
+                ("niter", C.c_int)]
+
+
 class FwTerm(C.Structure):
     _fields_ = [("name", C.c_char_p), ("penalizator", C.c_int), ("image_index", C.c_int),
                 ("image_to_add", C.c_int), ("prior", C.c_float), ("normalize", C.c_int)]
@@ -98,6 +106,7 @@ HIP_API = {
     "gvm_set_adjoint": [_P, _I],
     "gvm_chi2": [_P, _P, _I, _I, _DP],
     "gvm_chi2_enqueue": [_P, _P, _I, _I, _I],
+    "gvm_get_visibilities_out": [_P, _I, _P, _P, _P],
     "gvm_add_visibilities_dev": [_P, _F, _L, _P, _P, _P, _IP],
     "gvm_setup_beam_noise_dev": [_I, _P, _P, _P, _P, _D, _DP, _DP, _DP, _DP, _DP, _DP],
     "gvm_setup_beam_from_moments": [_D, _D, _D, _D, _D, _DP, _DP, _DP, _DP, _DP, _DP],
@@ -142,6 +151,9 @@ HIP_API = {
 }
 
 FW_API = {
+    "gvm_fits_write_image": ([C.c_char_p, _P, _I, _L, _L, _I, C.c_float, C.POINTER(FitsHeader)],
+                             _I),
+    "gvm_fits_last_error": ([], C.c_char_p),
     "gvm_sim_default_config": ([_I, C.POINTER(SimConfig)], None),
     "gvm_sim_count_visibilities": ([C.POINTER(SimConfig)], C.c_long),
     "gvm_sim_generate": ([C.POINTER(SimConfig), _P, _P, _P, _P, C.POINTER(SimComponent), _IP], _I),

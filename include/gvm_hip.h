@@ -175,6 +175,14 @@ gvm_status gvm_dchi2(gvm_ctx* ctx, const float* I_dev, float* result_dev,
 /* Residuals V_r of one slice from the last gvm_chi2 (host float[2*nvis]);
  * replaces the Vr readback of MFS::writeResiduals (src/mfs.cu:1115-1155). */
 gvm_status gvm_get_residuals(gvm_ctx* ctx, int slice, float* vr_host);
+/* The model and residual visibilities of one slice from the last gvm_chi2, in
+ * the frame of the observation: modelToHost (src/MSFITSIO.cu:1114-1138)
+ * conjugates V_m back for the rows that had u > 0 before the Hermitian flip
+ * (uvw_obs: the observed uvw, 3*nvis doubles, host; NULL when no row was
+ * flipped); the residuals V_o - V_m are conjugated likewise.  model_host /
+ * residual_host: 2*nvis floats each, either may be NULL. */
+gvm_status gvm_get_visibilities_out(gvm_ctx* ctx, int slice, const double* uvw_obs,
+                                    float* model_host, float* residual_host);
 
 /* ---- priors ------------------------------------------------------------- */
 /* Values: sum over the mask of the per-pixel term on image `index`, FP64.

@@ -94,3 +94,40 @@ def test_device_beam_noise_and_noise_image():
     assert np.array_equal(finite, np.isfinite(nd))
     assert np.allclose(nd[finite], nh[finite], rtol=2e-7)
     assert np.isclose(fd.value, fh.value, rtol=2e-7) and np.isclose(cd.value, ch.value, rtol=2e-7)
+
+
+def test_model_and_residual_output_in_observed_frame(tmp_path):
+    """modelToHost: V_m = V_o - V_r conjugated back for the rows flipped at
+    set-up; residuals likewise; and a FITS image written from the device."""
+    import oracle
+    from gpuvmem_amd.output import read_fits, write_fits
+
+    p = small_problem(npix=64, n_antennas=7, seed=9)
+    uvw, vo, w = _raw_rows(p)
+    assert (uvw[:, 0] > 0).any() and (uvw[:, 0] <= 0).any()
+    ctx = Context(p)  # the problem's rows are the flipped copies of these
+    img = p.initial_image()
+    img[0] += 5.0
+    d = ctx.image(img)
+    ctx.chi2(d)
+    model = np.zeros((p.nvis, 2), np.float32)
+    resid = np.zeros((p.nvis, 2), np.float32)
+    native.check(ctx.lib.gvm_get_visibilities_out(ctx.ctx, 0, uvw.ctypes.data, model.ctypes.data,
+                                                  resid.ctypes.data), "visibilities out")
+    _, vr, _ = oracle.Oracle(p).chi2(img)
+    flip = uvw[:, 0] > 0
+    vm_ref = (p.vo[:, 0] + 1j * p.vo[:, 1]) - vr  # flipped frame
+    vm_ref[flip] = np.conj(vm_ref[flip])
+    vr_obs = vr.copy()
+    vr_obs[flip] = np.conj(vr_obs[flip])
+    scale = np.abs(vm_ref).max()
+    assert np.abs(model[:, 0] + 1j * model[:, 1] - vm_ref).max() <= 1e-5 * scale
+    assert np.abs(resid[:, 0] + 1j * resid[:, 1] - vr_obs).max() <= 1e-5 * scale
+    # residuals + model reproduce the observed visibilities (the observed frame)
+    assert np.allclose(model + resid, vo, atol=1e-5 * scale)
+    path = str(tmp_path / "gpu.fits")
+    write_fits(path, d, p, index=0, niter=3)
+    _, data = read_fits(path)
+    clipped = d.download(np.float32, (2, p.N, p.M))
+    assert np.array_equal(data, clipped[0] * np.float32(p.fg_scale))
+    ctx.close()

@@ -41,7 +41,7 @@ def test_hip_library_exports_every_declared_symbol():
 
 def test_framework_library_exports_every_declared_symbol():
     exp = _exported(native.LIB_DIR / "libgvm.so")
-    for h in ("fw.h", "setup.h", "simdata.h"):
+    for h in ("fw.h", "setup.h", "simdata.h", "fitsio.h"):
         decl = _declared(ROOT / "include" / "gvm" / h)
         assert decl, h
         assert decl <= exp, (h, sorted(decl - exp))
@@ -225,3 +225,29 @@ def test_oracle_is_not_reachable_from_the_product():
         assert not re.search(r"^\s*(import|from)\s+oracle|liboracle|gvm_oracle", t, re.M), f
     ldd = subprocess.run(["ldd", str(native.LIB_DIR / "libgvm.so")], capture_output=True, text=True).stdout
     assert "oracle" not in ldd
+
+
+def test_fits_writer_round_trip(tmp_path):
+    """The FITS writer: fixed-format cards, 2880-byte records, big-endian
+    BITPIX -32 data equal to image * fg_scale, the keys OCopyFITS sets."""
+    from gpuvmem_amd.output import read_fits, write_fits
+
+    p = small_problem(npix=64)
+    rng = np.random.default_rng(4)
+    img = rng.standard_normal((2, p.N, p.M)).astype(np.float32)
+    path = str(tmp_path / "mem.fits")
+    write_fits(path, img, p, index=0, niter=37)
+    raw = open(path, "rb").read()
+    assert len(raw) % 2880 == 0
+    assert raw[:30].decode() == "SIMPLE  =                    T"
+    assert raw[80:110].decode() == "BITPIX  =                  -32"
+    hdr, data = read_fits(path)
+    assert (hdr["NAXIS1"], hdr["NAXIS2"], hdr["NITER"], hdr["BUNIT"]) == (p.M, p.N, 37, "JY/PIXEL")
+    assert hdr["CDELT1"] == pytest.approx(p.deltax, rel=1e-12) and hdr["CRPIX1"] == p.x0 + 1
+    assert hdr["EQUINOX"] == 2000.0 and hdr["RADESYS"] == "ICRS"
+    assert np.array_equal(data, img[0] * np.float32(p.fg_scale))
+    write_fits(path, img, p, index=1)  # alpha: unscaled, no unit
+    hdr, data = read_fits(path)
+    assert np.array_equal(data, img[1]) and hdr["BUNIT"] == ""
+    with pytest.raises(OSError):
+        write_fits(str(tmp_path / "no" / "dir" / "x.fits"), img, p)
