@@ -421,13 +421,15 @@ __global__ void adjoint_epilogue_kernel(const float* __restrict__ part,
                                         const SliceDev* __restrict__ tab, int nsl, long n,
                                         const unsigned int* __restrict__ cmaxs, int scaled,
                                         const float* __restrict__ noise, float noise_cut,
-                                        float fg_scale, int normalize, float threshold,
-                                        int flag_opt, const float* __restrict__ I,
+                                        float fg_scale, const float* __restrict__ gcf,
+                                        int normalize, float threshold, int flag_opt,
+                                        const float* __restrict__ I,
                                         float* __restrict__ result) {
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
        k += (long)gridDim.x * blockDim.x) {
     if (!(noise[k] < noise_cut)) continue;
     const float i0 = I[k], alpha = I[n + k];
+    const float gk = gcf ? gcf[k] : 1.0f;
     float r0 = result[k], r1 = result[n + k];
     for (int sl = 0; sl < nsl; sl++) {
       // FP64 sum over the slice's splits in split order; eight loads are
@@ -451,8 +453,11 @@ __global__ void adjoint_epilogue_kernel(const float* __restrict__ part,
       }
       const SliceDev& sd = tab[sl];
       const double inv_nvis = normalize ? 1.0 / (double)sd.nvis : 1.0;
-      // DChi2: dchi2 = -(sum * fg_scale * atten) [/ nvis]
-      const float d = (float)(-(g * inv_scale) * (double)(fg_scale * sd.pb[k]) * inv_nvis);
+      // DChi2: dchi2 = -(sum * fg_scale * atten [* gcf]) [/ nvis]
+      // (src/functions.cu:3698-3791; with gcf :3793-3888)
+      float sc = fg_scale * sd.pb[k];
+      if (gcf) sc = sc * gk;
+      const float d = (float)(-(g * inv_scale) * (double)sc * inv_nvis);
       const float dI = powf(sd.nudiv, alpha);
       if (flag_opt % 2 == 0) {
         r0 += d * dI;
@@ -599,7 +604,7 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
   hipLaunchKernelGGL(adjoint_epilogue_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      ctx->stream, ctx->adj_part, ctx->slice_p0, ctx->slice_tab, nsl, n, ctx->cmax,
                      mode == GVM_ADJOINT_DIRECT ? 0 : 1, ctx->noise, g.noise_cut, g.fg_scale,
-                     normalize, g.threshold, flag_opt, I, result);
+                     ctx->gcf, normalize, g.threshold, flag_opt, I, result);
   GVM_CHECK_LAUNCH();
   GVM_HIP_TRY(hipEventSynchronize(ctx->ev_b));
   float ms = 0.f;

@@ -277,6 +277,7 @@ extern "C" gvm_status gvm_destroy(gvm_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& s : ctx->slices) free_slice(s);
   (void)hipFree(ctx->noise);
+  if (ctx->gcf) (void)hipFree(ctx->gcf);
   (void)hipFree(ctx->grid);
   (void)hipFree(ctx->twiddle);
   (void)hipFree(ctx->rot_x);

@@ -153,15 +153,15 @@ __device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ 
   }
 }
 
-// Row pass: build I_nu * PB * fg_scale for B rows of slice blockIdx.y
+// Row pass: build I_nu * PB * fg_scale [* GCF] for B rows of slice blockIdx.y
 // (calculateInu + apply_beam2I, src/functions.cu:3939-3966, 2424-2444),
 // transform each row, multiply by rot_x[column frequency], write the rows of
 // that slice's grid.
 template <int N, int B>
 __global__ __launch_bounds__(256) void fft_rows_kernel(
     const float* __restrict__ I, const SliceDev* __restrict__ tab, float fg_scale,
-    float floor_v, const float2* __restrict__ tw, const float2* __restrict__ rot_x,
-    float2* __restrict__ grids) {
+    float floor_v, const float* __restrict__ gcf, const float2* __restrict__ tw,
+    const float2* __restrict__ rot_x, float2* __restrict__ grids) {
   __shared__ float2 lds[B * N];
   const long n = (long)N * N;
   const float* __restrict__ pb = tab[blockIdx.y].pb;
@@ -172,7 +172,9 @@ __global__ __launch_bounds__(256) void fft_rows_kernel(
     const long k = (long)(row0 + idx / N) * N + (idx % N);
     float inu = I[k] * powf(nudiv, I[n + k]);
     if (inu < floor_v) inu = floor_v;
-    lds[idx] = make_float2(inu * pb[k] * fg_scale, 0.0f);
+    float x = inu * pb[k] * fg_scale;
+    if (gcf) x = x * gcf[k];  // apply_GCF (src/functions.cu:2468-2476)
+    lds[idx] = make_float2(x, 0.0f);
   }
   __syncthreads();
   fft_lds<N, B, false>(lds, tw);
@@ -208,8 +210,8 @@ gvm_status launch_fft(gvm_ctx* ctx, const float* I, int nsl, float floor_v) {
   constexpr int BC0 = 8192 / N;
   constexpr int BC = BC0 < 2 ? 2 : (BC0 > N ? N : BC0);
   hipLaunchKernelGGL((fft_rows_kernel<N, BR>), dim3(N / BR, nsl), dim3(256), 0, ctx->stream, I,
-                     ctx->slice_tab, ctx->g.fg_scale, floor_v, ctx->twiddle, ctx->rot_x,
-                     ctx->grid);
+                     ctx->slice_tab, ctx->g.fg_scale, floor_v, ctx->gcf, ctx->twiddle,
+                     ctx->rot_x, ctx->grid);
   GVM_CHECK_LAUNCH();
   hipLaunchKernelGGL((fft_cols_kernel<N, BC>), dim3(N / BC, nsl), dim3(256), 0, ctx->stream,
                      ctx->grid, ctx->twiddle, ctx->rot_y);

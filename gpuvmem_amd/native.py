@@ -12,9 +12,11 @@ from pathlib import Path
 LIB_DIR = Path(__file__).resolve().parent / "lib"
 
 GVM_OK = 0
+GVM_ERR_ARG = 1
 GVM_BEAM_AIRYDISK = 0
 GVM_BEAM_GAUSSIAN = 1
 ADJOINT_MODES = {"f32": 0, "f16x3": 1, "bf16x3": 2, "direct": 3}
+WEIGHTING_SCHEMES = {"natural": 0, "uniform": 1, "briggs": 2, "radial": 3}
 
 
 class GvmError(RuntimeError):
@@ -34,6 +36,17 @@ class Geometry(C.Structure):
         ("beam_kind", C.c_int), ("antenna_diameter", C.c_float),
         ("pb_factor", C.c_float), ("pb_cutoff", C.c_float),
     ]
+
+
+class CKernel(C.Structure):
+    """gvm_ckernel: m x n taps (host pointer) with supports."""
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("support_x", C.c_int), ("support_y", C.c_int),
+                ("taps", C.c_void_p)]
+
+
+class UVTaper(C.Structure):
+    _fields_ = [("amplitude", C.c_float), ("u_0", C.c_double), ("v_0", C.c_double),
+                ("sigma_maj", C.c_float), ("sigma_min", C.c_float), ("bpa", C.c_float)]
 
 
 class SimConfig(C.Structure):
@@ -108,6 +121,12 @@ HIP_API = {
     "gvm_chi2_enqueue": [_P, _P, _I, _I, _I],
     "gvm_get_visibilities_out": [_P, _I, _P, _P, _P],
     "gvm_add_visibilities_dev": [_P, _F, _L, _P, _P, _P, _IP],
+    "gvm_grid_visibilities_dev": [_P, C.POINTER(CKernel), _F, _L, _P, _P, _P, _P, _P, _P, _L,
+                                  C.POINTER(_L)],
+    "gvm_add_visibilities_gridded_dev": [_P, C.POINTER(CKernel), _F, _L, _P, _P, _P,
+                                         C.POINTER(_L), _IP],
+    "gvm_set_gcf": [_P, _P],
+    "gvm_weighting_apply_dev": [_P, _I, _F, C.POINTER(UVTaper), _I, _P, _P, _P, _P],
     "gvm_setup_beam_noise_dev": [_I, _P, _P, _P, _P, _D, _DP, _DP, _DP, _DP, _DP, _DP],
     "gvm_setup_beam_from_moments": [_D, _D, _D, _D, _D, _DP, _DP, _DP, _DP, _DP, _DP],
     "gvm_setup_noise_image_dev": [_I, _D, _D, _D, _D, _I, _I, _D, _D, _F, _F, _D, _D, _P,

@@ -116,6 +116,78 @@ int gvm_setup_noise_image_dev(int beam_kind, double diameter, double pb_factor,
                               double noise_cut_factor, float* noise_out, float* fg_scale,
                               float* noise_cut);
 
+/* ---- gridded-visibility mode (set-up) ----------------------------------- */
+/* A convolution (anti-aliasing) kernel as the reference's CKernel holds it
+ * (include/classes/ckernel.cuh:415-511): m rows by n columns of taps,
+ * taps[n*i + j], centred at (support_y, support_x).  The reference's default
+ * is PillBox2D() (src/pillBox2D.cu:22-25): m = n = 1, support 0, tap 1. */
+typedef struct {
+  int m, n;
+  int support_x, support_y;
+  const float* taps; /* host, m*n floats */
+} gvm_ckernel;
+
+/* Convolutional gridding of one channel x stokes slice onto the M x N uv
+ * grid of ctx: do_gridding (src/functions.cu:1339-1653) for one slice.  Each
+ * row and its Hermitian twin (u, v, w and Im Vo negated) is placed at the
+ * cell int(u / deltau + floor(N/2) + 0.5) (u in wavelengths, FP64) and spread
+ * over the kernel's taps; per cell, sum w k, sum w k^2 and sum w k Vo are
+ * accumulated in float in row order (the reference's order with one gridding
+ * thread), then Vo = sum w k Vo / sum w k and weight = (sum w k)^2 /
+ * sum w k^2.  Cells with a positive weight are written in row-major cell
+ * order as visibilities at the cell centre (uvw in metres, w = 0).
+ * uvw_m_dev / vo_dev / weight_dev: the raw rows on the device (metres,
+ * before the Hermitian flip; 3, 2 and 1 values per row); the outputs hold
+ * `capacity` rows (M*N always suffices); *nout receives the count.
+ * Synchronises the device. */
+gvm_status gvm_grid_visibilities_dev(gvm_ctx* ctx, const gvm_ckernel* kernel, float freq,
+                                     long nvis, const double* uvw_m_dev, const float* vo_dev,
+                                     const float* weight_dev, double* uvw_out_dev,
+                                     float* vo_out_dev, float* weight_out_dev, long capacity,
+                                     long* nout);
+/* The gridded rows of one slice added to ctx as a new slice: the
+ * do_gridding step of MFS::configure (src/mfs.cu:510-527) followed by the
+ * device set-up of gvm_add_visibilities_dev.  *ngridded receives the number
+ * of gridded rows (may be NULL). */
+gvm_status gvm_add_visibilities_gridded_dev(gvm_ctx* ctx, const gvm_ckernel* kernel, float freq,
+                                            long nvis, const double* uvw_m_dev,
+                                            const float* vo_dev, const float* weight_dev,
+                                            long* ngridded, int* slice_out);
+/* The gridding correction function image (CKernel::getGCFGPU, host float[M*N],
+ * or NULL for none): once set, the model image is multiplied by it before
+ * the transform (apply_GCF, src/functions.cu:2468-2476, called at :4226-4229
+ * and :4358-4362) and the gradient by it per pixel (the DChi2 variant with
+ * gcf, src/functions.cu:3793-3888). */
+gvm_status gvm_set_gcf(gvm_ctx* ctx, const float* gcf_host);
+
+/* Weighting schemes (the WeightingScheme::apply plugins run by
+ * MFS::configure before gridding, src/mfs.cu:499-508). */
+#define GVM_WEIGHTING_NATURAL 0 /* src/naturalweightingscheme.cu:9-48 */
+#define GVM_WEIGHTING_UNIFORM 1 /* src/uniformweightingscheme.cu:9-99 */
+#define GVM_WEIGHTING_BRIGGS 2  /* src/briggsweightingscheme.cu:30-187 */
+#define GVM_WEIGHTING_RADIAL 3  /* src/radialweightingscheme.cu:8-43 */
+
+/* UVTaper (include/classes/uvtaper.cuh): amplitude * exp(-a x^2 - b x y -
+ * c y^2) at (x, y) = (u - u_0, v - v_0) in wavelengths. */
+typedef struct {
+  float amplitude;
+  double u_0, v_0;
+  float sigma_maj, sigma_min, bpa;
+} gvm_uvtaper;
+
+/* Applies `scheme` to the weights of nsl slices in place on the device
+ * (weight_dev[s]: nvis[s] floats; uvw_m_dev[s]: raw rows in metres), with
+ * the grid of ctx (cells of |deltau| x |deltav|, rows with u < 0 mirrored).
+ * Uniform: w / (sum of w in the cell); Briggs: w / (1 + G f^2) with f^2 =
+ * (5 10^-robust)^2 / (sum G^2 / sum w), robust in [-2, 2]; Radial: w * |uv|;
+ * then each scheme multiplies by the taper (NULL: none).  The per-cell sums
+ * are float sums in row order, as the reference's with one thread; the two
+ * Briggs totals are FP64 sums (the reference sums them in float). */
+gvm_status gvm_weighting_apply_dev(gvm_ctx* ctx, int scheme, float robust,
+                                   const gvm_uvtaper* taper, int nsl, const float* freq,
+                                   const long* nvis, const double* const* uvw_m_dev,
+                                   float* const* weight_dev);
+
 /* Replaces MFS::unSetDevice (src/mfs.cu:1157-1253). */
 gvm_status gvm_destroy(gvm_ctx* ctx);
 

@@ -21,6 +21,12 @@
 /* first zero of J1 (reference src/functions.cu AiryDiskBeam uses RZ) */
 #define ORC_RZ 1.2196698912665045
 
+/* The gridding correction factor of pixel k: apply_GCF and the DChi2 variant
+ * with gcf (src/functions.cu:2468-2476, 3793-3888); 1 without gridding. */
+static inline double gcf_at(const orc_geom* g, long k) {
+  return g->gcf ? (double)g->gcf[k] : 1.0;
+}
+
 /* sin/cos of 2*pi*t with the argument reduced exactly in turns first. */
 static inline void sincos_turns(double t, double* s, double* c) {
   double r = t - nearbyint(t);
@@ -260,7 +266,7 @@ void orc_model_grid(const orc_geom* g, const orc_beam* b, const double* I,
       if (inu < floor_v) inu = floor_v;
       double pb = orc_attenuation(b, freq, i, j, g->x0_ref, g->y0_ref,
                                   g->deltax, g->deltay);
-      grid[k].re = inu * pb * (double)g->fg_scale;
+      grid[k].re = inu * pb * (double)g->fg_scale * gcf_at(g, k);
       grid[k].im = 0.0;
     }
   }
@@ -379,7 +385,7 @@ void orc_dchi2_direct_pixels(const orc_geom* g, const orc_beam* b,
     }
     double atten = orc_attenuation(b, ch->freq, i, j, g->x0_ref, g->y0_ref,
                                    g->deltax, g->deltay);
-    double d = pixel_dchi2(g, ch, vr, i, j) * (double)g->fg_scale * atten;
+    double d = pixel_dchi2(g, ch, vr, i, j) * (double)g->fg_scale * atten * gcf_at(g, k);
     if (normalize) d /= (double)ch->nvis;
     out[p] = -d;
   }
@@ -407,7 +413,7 @@ void orc_dchi2_channel(const orc_geom* g, const orc_beam* b,
         }
         double atten = orc_attenuation(b, ch->freq, i, j, g->x0_ref, g->y0_ref,
                                        g->deltax, g->deltay);
-        double d = pixel_dchi2(g, ch, vr, i, j) * (double)g->fg_scale * atten;
+        double d = pixel_dchi2(g, ch, vr, i, j) * (double)g->fg_scale * atten * gcf_at(g, k);
         if (normalize) d /= (double)ch->nvis;
         dchi2[k] = -d;
       }
@@ -475,7 +481,7 @@ void orc_dchi2_channel(const orc_geom* g, const orc_beam* b,
       }
       double atten = orc_attenuation(b, ch->freq, i, j, g->x0_ref, g->y0_ref,
                                      g->deltax, g->deltay);
-      double d = acc[k] * (double)g->fg_scale * atten;
+      double d = acc[k] * (double)g->fg_scale * atten * gcf_at(g, k);
       if (normalize) d /= (double)ch->nvis;
       dchi2[k] = -d;
     }

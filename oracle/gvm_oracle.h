@@ -46,6 +46,7 @@ typedef struct {
   float threshold; /* -T, default 0 */
   float nu_0;
   const float* noise;
+  const float* gcf; /* gridding correction image (M*N floats) or NULL */
 } orc_geom;
 
 typedef struct {
@@ -64,6 +65,23 @@ typedef struct {
   const float* vo;   /* 2*nvis */
   float* weight;     /* nvis */
 } orc_chan;
+
+/* UVTaper (include/classes/uvtaper.cuh). */
+typedef struct {
+  float amplitude;
+  double u_0, v_0;
+  float sigma_maj, sigma_min, bpa;
+} orc_taper;
+
+/* ---- gridded-visibility mode (gvm_oracle_grid.c) ---- */
+void orc_grid_visibilities(int M, int N, double deltax_deg, double deltay_deg, float freq,
+                           long nvis, const double* uvw_m, const float* vo, const float* weight,
+                           int km, int kn, int supx, int supy, const float* taps,
+                           double* out_uvw, float* out_vo, float* out_w, long* nout);
+void orc_weighting(int scheme, float robust, const orc_taper* taper, int M, int N,
+                   double deltax_deg, double deltay_deg, int nsl, const float* freq,
+                   const long* nvis, const double* const* uvw_m, float* const* weight,
+                   int float_totals);
 
 /* ---- set-up (row a19) ---- */
 void orc_hermitian_to_lambda(long n, double* uvw, float* vo, float freq);

@@ -31,8 +31,13 @@ class OGeom(C.Structure):
         ("x0_phs", C.c_float), ("y0_phs", C.c_float),
         ("fg_scale", C.c_float), ("noise_cut", C.c_float), ("minpix", C.c_float),
         ("eta", C.c_float), ("threshold", C.c_float), ("nu_0", C.c_float),
-        ("noise", C.c_void_p),
+        ("noise", C.c_void_p), ("gcf", C.c_void_p),
     ]
+
+
+class OTaper(C.Structure):
+    _fields_ = [("amplitude", C.c_float), ("u_0", C.c_double), ("v_0", C.c_double),
+                ("sigma_maj", C.c_float), ("sigma_min", C.c_float), ("bpa", C.c_float)]
 
 
 class OBeam(C.Structure):
@@ -126,6 +131,10 @@ def lib() -> C.CDLL:
             "orc_frprmn": ([C.POINTER(OProblem), P, I, D, D, C.POINTER(OResult), P, I], I),
             "orc_lbfgs": ([C.POINTER(OProblem), P, I, I, D, D, C.POINTER(OResult), P, I], I),
             "orc_do_gridding_pillbox": ([I, I, D, D, CP, I, I, I, P, P, P], None),
+            "orc_grid_visibilities": ([I, I, D, D, C.c_float, C.c_long, P, P, P, I, I, I, I, P,
+                                       P, P, P, P], None),
+            "orc_weighting": ([I, C.c_float, C.POINTER(OTaper), I, I, D, D, I, P, P, P, P, I],
+                              None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -137,6 +146,47 @@ def lib() -> C.CDLL:
 
 def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
+
+
+def grid_visibilities(M: int, N: int, deltax: float, deltay: float, freq: float, uvw_m, vo, weight,
+                      taps, support_x: int, support_y: int):
+    """do_gridding of one slice (gvm_oracle_grid.c): returns the gridded
+    (uvw metres (n, 3) float64, vo (n, 2) float32, weight (n,) float32).
+    taps: (m, n) float32 kernel."""
+    uvw_m = np.ascontiguousarray(uvw_m, np.float64)
+    vo = np.ascontiguousarray(vo, np.float32)
+    weight = np.ascontiguousarray(weight, np.float32)
+    taps = np.ascontiguousarray(taps, np.float32)
+    cap = M * N
+    ou = np.zeros((cap, 3), np.float64)
+    ov = np.zeros((cap, 2), np.float32)
+    ow = np.zeros(cap, np.float32)
+    nout = C.c_long()
+    lib().orc_grid_visibilities(M, N, deltax, deltay, freq, uvw_m.shape[0], _ptr(uvw_m), _ptr(vo),
+                                _ptr(weight), taps.shape[0], taps.shape[1], support_x, support_y,
+                                _ptr(taps), _ptr(ou), _ptr(ov), _ptr(ow), C.byref(nout))
+    n = nout.value
+    return ou[:n].copy(), ov[:n].copy(), ow[:n].copy()
+
+
+WEIGHTING = {"natural": 0, "uniform": 1, "briggs": 2, "radial": 3}
+
+
+def weighting(scheme: str, slices, M: int, N: int, deltax: float, deltay: float,
+              robust: float = 0.0, taper=None, float_totals: bool = True):
+    """WeightingScheme::apply over slices [(freq, uvw_m, weight)]: returns
+    the new weight arrays (float32).  taper: dict of OTaper fields or None."""
+    nsl = len(slices)
+    freqs = (C.c_float * nsl)(*[s[0] for s in slices])
+    nv = (C.c_long * nsl)(*[s[1].shape[0] for s in slices])
+    us = [np.ascontiguousarray(s[1], np.float64) for s in slices]
+    ws = [np.ascontiguousarray(s[2], np.float32).copy() for s in slices]
+    up = (C.c_void_p * nsl)(*[_ptr(u) for u in us])
+    wp = (C.c_void_p * nsl)(*[_ptr(w) for w in ws])
+    t = OTaper(**taper) if taper is not None else None
+    lib().orc_weighting(WEIGHTING[scheme], robust, C.byref(t) if t is not None else None, M, N,
+                        deltax, deltay, nsl, freqs, nv, up, wp, int(float_totals))
+    return ws
 
 
 def fft2d(a: np.ndarray, sign: int = +1) -> np.ndarray:
@@ -168,19 +218,21 @@ class Oracle:
     ``noise`` defaults to the problem's float32 noise image so the mask is
     bit-identical to the one the device uses."""
 
-    def __init__(self, problem, separable: bool = True, noise=None, slices=None):
+    def __init__(self, problem, separable: bool = True, noise=None, slices=None, gcf=None):
         """slices: optional list of (freq, uvw, vo, weight) replacing the
-        problem's single channel."""
+        problem's single channel; gcf: optional gridding correction image."""
         self.p = problem
         L = lib()
         self.noise = np.ascontiguousarray(problem.noise if noise is None else noise, np.float32)
+        self.gcf = None if gcf is None else np.ascontiguousarray(gcf, np.float32)
         dxr = problem.deltax * np.pi / 180.0
         dyr = problem.deltay * np.pi / 180.0
         self.geom = OGeom(problem.M, problem.N, problem.deltax, problem.deltay,
                           1.0 / (problem.M * dxr), 1.0 / (problem.N * dyr),
                           problem.x0, problem.y0, problem.x0, problem.y0,
                           problem.fg_scale, problem.noise_cut, problem.minpix, problem.eta,
-                          problem.threshold, problem.nu_0, _ptr(self.noise))
+                          problem.threshold, problem.nu_0, _ptr(self.noise),
+                          _ptr(self.gcf) if self.gcf is not None else None)
         self.beam = OBeam(problem.beam_kind, problem.antenna_diameter, problem.pb_factor,
                           problem.pb_cutoff)
         if slices is None:
