@@ -112,6 +112,73 @@ def gridding_comparator(p, oracle, threads: int, k: int = 100000):
     return (time.perf_counter() - t0) * p.nvis_total / k, k
 
 
+def bench_gridding(p, steps: int, warmup: int, threads: int, adjoint: str) -> dict:
+    """The gridded-visibility branch of the set-up on the device (Briggs
+    weighting, then gridding) timed on the workload's rows, beside the CPU
+    restatement of the reference's OpenMP gridder, and the objective step on
+    the gridded rows."""
+    from gpuvmem_amd.device import Context, DeviceBuffer, Objective, fw_config
+    from gpuvmem_amd.gridding import GriddingKernel, add_gridded_dev, apply_weighting_dev
+    from gpuvmem_amd.gridding import grid_visibilities_dev
+
+    lam = float(np.float32(2.99792458e8) / np.float32(p.freq))
+    uvw = np.ascontiguousarray(p.uvw * lam)  # metres (the flip does not change the gridding)
+    n = uvw.shape[0]
+    ctx = Context(p, adjoint=adjoint, add_data=False)
+    bu, bv = DeviceBuffer.from_array(uvw), DeviceBuffer.from_array(p.vo)
+    bw = DeviceBuffer.from_array(p.weight)
+    out = {}
+    # Briggs weighting (robust 0): two passes over the rows with sorted cells
+    ts = []
+    for _ in range(3):
+        bw.upload(p.weight)
+        t0 = time.perf_counter()
+        apply_weighting_dev(ctx, "briggs", [(p.freq, bu, bw, n)], robust=0.0)
+        ts.append(time.perf_counter() - t0)
+    out["briggs_weighting_ms"] = min(ts) * 1e3
+    bw.upload(p.weight)
+    for name, k in (("pillbox_1x1", GriddingKernel.pillbox()),
+                    ("pillbox_7x7", GriddingKernel.build("pillbox", p.M, p.N, p.deltax, p.deltay,
+                                                         m=7, n=7))):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            gu, _, _ = grid_visibilities_dev(ctx, k, p.freq, bu, bv, bw, n)
+            ts.append(time.perf_counter() - t0)
+        out[name] = {"ms": min(ts) * 1e3, "rows_per_s": n / min(ts), "gridded_rows": gu.shape[0],
+                     "note": "includes copying the gridded rows to the host"}
+    if threads > 0:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        import oracle
+
+        cpu_s, k = gridding_comparator(p, oracle, threads)
+        out["cpu_comparator_7x7"] = {"seconds": cpu_s, "threads": threads,
+                                     "sample": f"{k} of {n} rows, extrapolated linearly"}
+        out["speedup_7x7"] = cpu_s / (out["pillbox_7x7"]["ms"] * 1e-3)
+    # the objective step on the gridded rows (default 1x1 pillbox)
+    s, ng = add_gridded_dev(ctx, GriddingKernel.pillbox(), p.freq, bu, bv, bw, n)
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
+    obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
+    d_img = ctx.image(bench_image(p))
+    xi = ctx.zeros(2 * p.M * p.N)
+    for _ in range(warmup):
+        obj.value(d_img)
+        obj.gradient(d_img, xi, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        obj.value(d_img)
+        obj.gradient(d_img, xi, 1)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    out["gridded_objective"] = {"rows": ng, "evals_per_s": steps / el,
+                                "ms_per_step": el / steps * 1e3, "adjoint": adjoint}
+    obj.close()
+    ctx.close()
+    return out
+
+
 def profiled_traffic(mode: str, nvis: int):
     """HBM bytes per launch of the adjoint GEMM from the committed rocprofv3
     counter summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
@@ -266,6 +333,8 @@ def main() -> int:
     ap.add_argument("--precision-study", action="store_true",
                     help="instead of evals/s: each adjoint form against FP64 (use with "
                          "--workload 2: 2048^2, 4.35M vis)")
+    ap.add_argument("--gridding", action="store_true",
+                    help="time the device weighting and gridder and the gridded objective step")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size (for counter scaling)")
     args = ap.parse_args()
@@ -286,6 +355,13 @@ def main() -> int:
         r = precision_study(p)
         print(json.dumps({"metric": f"adjoint precision against FP64 ({p.M}^2, {p.nvis_total} vis)",
                           "n_gpus": 1, "data": "synthetic", **r}), flush=True)
+        return 0
+    if args.gridding:
+        threads = 0 if args.no_cpu_baseline else min(16, os.cpu_count() or 1)
+        r = bench_gridding(p, args.steps, args.warmup, threads, args.adjoint)
+        print(json.dumps({"metric": f"gridded-visibility set-up and step ({p.M}^2, "
+                                    f"{p.nvis_total} vis)", "n_gpus": 1, "data": "synthetic",
+                          **r}), flush=True)
         return 0
     if args.time_to_image > 0:
         r = time_to_image(p, args.time_to_image)

@@ -25,7 +25,7 @@ INCLUDE = ROOT / "include"
 HIP_SOURCES = ["gvm_ctx.hip", "gvm_forward.hip", "gvm_adjoint.hip", "gvm_ops.hip", "gvm_setup.hip",
                "gvm_grid.hip"]
 HOST_SOURCES = ["framework.cpp", "optimizers.cpp", "setup.cpp", "fw_capi.cpp", "simdata.cpp",
-                "fitsio.cpp"]
+                "fitsio.cpp", "ckernels.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -67,7 +67,9 @@ def build_host() -> Path:
     LIB.mkdir(exist_ok=True)
     out = LIB / "libgvm.so"
     tmp = LIB / "libgvm.so.tmp"
-    cmd = [CXX, "-std=c++17", "-O2", "-fPIC", "-fopenmp", "-shared", "-Wall", f"-I{INCLUDE}"]
+    # no multiply-add contraction: host float arithmetic in source order
+    cmd = [CXX, "-std=c++17", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-Wall",
+           f"-I{INCLUDE}"]
     cmd += [str(CSRC / "host" / s) for s in HOST_SOURCES]
     cmd += [f"-L{LIB}", "-lgvm_hip", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
     _run(cmd)

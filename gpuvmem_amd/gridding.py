@@ -40,6 +40,34 @@ class GriddingKernel:
         """PillBox2D() (src/pillBox2D.cu:22-25): 1 x 1, tap 1, GCF 1."""
         return cls.from_taps(np.ones((1, 1), np.float32), "Pill Box")
 
+    @classmethod
+    def build(cls, kind: str, M: int, N: int, deltax: float, deltay: float,
+              with_gcf: bool = True, **params) -> "GriddingKernel":
+        """A kernel of the reference's CKernel family built as MFS::configure
+        does (src/mfs.cu:510-516): taps with sigmas (|deltau|, |deltav|), and
+        the correction image for an M x N image of cells (deltax, deltay)
+        degrees.  params override the default constructor's m, n, amp, w,
+        w2, alpha.  Sinc2D and GaussianSinc2D have no correction image."""
+        fw = native.load_fw()
+        p = native.CKernelParams()
+        if fw.gvm_ckernel_defaults(native.CKERNEL_KINDS[kind], C.byref(p)) != 0:
+            raise ValueError(f"unknown kernel {kind}")
+        for k, v in params.items():
+            setattr(p, k, v)
+        dxr, dyr = deltax * np.pi / 180.0, deltay * np.pi / 180.0
+        du, dv = 1.0 / (M * dxr), 1.0 / (N * dyr)
+        taps = np.zeros((p.m, p.n), np.float32)
+        sx, sy = C.c_int(), C.c_int()
+        if fw.gvm_ckernel_build(C.byref(p), abs(du), abs(dv), taps.ctypes.data, C.byref(sx),
+                                C.byref(sy)) != 0:
+            raise ValueError("gvm_ckernel_build failed")
+        gcf = None
+        if with_gcf:
+            gcf = np.zeros((M, N), np.float32)
+            if fw.gvm_ckernel_gcf(C.byref(p), M, N, abs(dxr), abs(dyr), gcf.ctypes.data) != 0:
+                raise ValueError(f"the {kind} kernel has no correction image")
+        return cls(taps, sx.value, sy.value, kind, gcf)
+
     def as_c(self) -> native.CKernel:
         k = native.CKernel()
         k.m, k.n = self.taps.shape
@@ -108,3 +136,19 @@ def set_gcf(ctx, gcf: Optional[np.ndarray]) -> None:
     if g.size != ctx.n:
         raise ValueError("gcf must hold M*N values")
     native.check(ctx.lib.gvm_set_gcf(ctx.ctx, g.ctypes.data), "gvm_set_gcf")
+
+
+def configure_gridded(ctx, kernel: GriddingKernel, slices: Sequence, scheme: str = "natural",
+                      robust: float = 0.0, taper: Optional[dict] = None) -> list:
+    """The gridded branch of MFS::configure (src/mfs.cu:499-527) on device
+    rows: the weighting scheme over all slices, then each slice gridded with
+    `kernel` and added to ctx, then the kernel's correction image.  slices =
+    [(freq, uvw_m buffer, vo buffer, weight buffer, nvis)] (raw rows in
+    metres; the weight buffers are updated in place).  Returns the gridded
+    row count of each slice."""
+    apply_weighting_dev(ctx, scheme, [(f, u, w, n) for f, u, _, w, n in slices], robust, taper)
+    counts = []
+    for f, u, v, w, n in slices:
+        counts.append(add_gridded_dev(ctx, kernel, f, u, v, w, n)[1])
+    set_gcf(ctx, kernel.gcf)
+    return counts

@@ -44,6 +44,15 @@ class CKernel(C.Structure):
                 ("taps", C.c_void_p)]
 
 
+class CKernelParams(C.Structure):
+    """gvm_ckernel_params (include/gvm/ckernel.h)."""
+    _fields_ = [("kind", C.c_int), ("m", C.c_int), ("n", C.c_int), ("amp", C.c_float),
+                ("w", C.c_float), ("w2", C.c_float), ("alpha", C.c_float)]
+
+
+CKERNEL_KINDS = {"pillbox": 0, "gaussian": 1, "sinc": 2, "gaussian_sinc": 3, "pswf": 4}
+
+
 class UVTaper(C.Structure):
     _fields_ = [("amplitude", C.c_float), ("u_0", C.c_double), ("v_0", C.c_double),
                 ("sigma_maj", C.c_float), ("sigma_min", C.c_float), ("bpa", C.c_float)]
@@ -170,6 +179,9 @@ HIP_API = {
 }
 
 FW_API = {
+    "gvm_ckernel_defaults": ([_I, C.POINTER(CKernelParams)], _I),
+    "gvm_ckernel_build": ([C.POINTER(CKernelParams), C.c_float, C.c_float, _P, _IP, _IP], _I),
+    "gvm_ckernel_gcf": ([C.POINTER(CKernelParams), _I, _I, C.c_float, C.c_float, _P], _I),
     "gvm_fits_write_image": ([C.c_char_p, _P, _I, _L, _L, _I, C.c_float, C.POINTER(FitsHeader)],
                              _I),
     "gvm_fits_last_error": ([], C.c_char_p),

@@ -11,7 +11,7 @@ import oracle
 from gpuvmem_amd import native
 from gpuvmem_amd.device import Context, DeviceBuffer
 from gpuvmem_amd.gridding import (GriddingKernel, add_gridded_dev, apply_weighting_dev,
-                                  grid_visibilities_dev, set_gcf)
+                                  configure_gridded, grid_visibilities_dev, set_gcf)
 from gpuvmem_amd.problem import sim_config, small_problem
 
 pytestmark = pytest.mark.gpu
@@ -91,15 +91,18 @@ def test_gridder_rejects_small_capacity_and_grids_empty_input():
     ctx.close()
 
 
+def _cfg_at(p, f):
+    cfg = sim_config(0, **{k: getattr(p.sim, k) for k in ("npix", "n_antennas", "ha_start_hours",
+                                                          "ha_end_hours", "dump_seconds",
+                                                          "cell_arcsec", "seed_layout")})
+    cfg.freq_hz = float(f)
+    return cfg
+
+
 def _slices(p, freqs):
     out = []
     for f in freqs:
-        cfg = sim_config(0, **{k: getattr(p.sim, k) for k in ("npix", "n_antennas",
-                                                              "ha_start_hours", "ha_end_hours",
-                                                              "dump_seconds", "cell_arcsec",
-                                                              "seed_layout")})
-        cfg.freq_hz = float(f)
-        uvw, vo, w = _raw_rows(cfg)
+        uvw, vo, w = _raw_rows(_cfg_at(p, f))
         out.append((float(np.float32(f)), uvw, w))
     return out
 
@@ -203,3 +206,44 @@ def test_unit_gcf_changes_nothing():
         ctx.close()
     assert out[0][0] == out[1][0]
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("kind,params", [("pillbox", {}), ("gaussian", {"w": 3.0}),
+                                         ("pswf", {})])
+def test_configure_gridded_with_built_kernels(kind, params):
+    """Briggs weighting, gridding with a kernel of the reference's family and
+    its correction image, over two channels, against the oracle."""
+    p = small_problem(npix=128, n_antennas=10, seed=17)
+    k = GriddingKernel.build(kind, p.M, p.N, p.deltax, p.deltay, **params)
+    sl = _slices(p, [226e9, 234e9])
+    vos = [_raw_rows(_cfg_at(p, f))[1] for f, _, _ in sl]
+    print(f"case: configure gridded {kind} {p.M}^2 nvis={[s[1].shape[0] for s in sl]}", flush=True)
+    ctx = Context(p, adjoint="f16x3", add_data=False)
+    bufs = [(f, DeviceBuffer.from_array(u), DeviceBuffer.from_array(v), DeviceBuffer.from_array(w),
+             u.shape[0]) for (f, u, w), v in zip(sl, vos)]
+    counts = configure_gridded(ctx, k, bufs, scheme="briggs", robust=0.0)
+    ws = oracle.weighting("briggs", sl, p.M, p.N, p.deltax, p.deltay, robust=0.0,
+                          float_totals=False)
+    oslices = []
+    for (f, u, _), v, w in zip(sl, vos, ws):
+        gu, gv, gw = oracle.grid_visibilities(p.M, p.N, p.deltax, p.deltay, f, u, v, w, k.taps,
+                                              k.support_x, k.support_y)
+        oracle.lib().orc_hermitian_to_lambda(gu.shape[0], gu.ctypes.data, gv.ctypes.data, f)
+        oslices.append((f, gu, gv, gw))
+    assert counts == [s[1].shape[0] for s in oslices]
+    orc = oracle.Oracle(p, slices=oslices, gcf=k.gcf)
+    img = p.initial_image()
+    img[0] += (np.random.default_rng(3).random((p.N, p.M)) * 10).astype(np.float32)
+    d = ctx.image(img)
+    v = ctx.chi2(d)
+    ref, vr, clipped = orc.chi2(img)
+    r = ctx.zeros(2 * p.M * p.N)
+    ctx.dchi2(d, r)
+    ctx.sync()
+    g = r.download(np.float32, (2, p.N, p.M))
+    gref = orc.dchi2(clipped, vr)
+    err = np.linalg.norm(g[0] - gref[0]) / np.linalg.norm(gref[0])
+    print(f"chi2 rel {abs(v - ref) / abs(ref):.2e} grad rel L2 {err:.2e} rows {counts}")
+    assert abs(v - ref) <= 1e-6 * abs(ref)
+    assert err < 1e-5
+    ctx.close()

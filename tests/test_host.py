@@ -41,7 +41,7 @@ def test_hip_library_exports_every_declared_symbol():
 
 def test_framework_library_exports_every_declared_symbol():
     exp = _exported(native.LIB_DIR / "libgvm.so")
-    for h in ("fw.h", "setup.h", "simdata.h", "fitsio.h"):
+    for h in ("fw.h", "setup.h", "simdata.h", "fitsio.h", "ckernel.h"):
         decl = _declared(ROOT / "include" / "gvm" / h)
         assert decl, h
         assert decl <= exp, (h, sorted(decl - exp))
@@ -251,3 +251,73 @@ def test_fits_writer_round_trip(tmp_path):
     assert np.array_equal(data, img[1]) and hdr["BUNIT"] == ""
     with pytest.raises(OSError):
         write_fits(str(tmp_path / "no" / "dir" / "x.fits"), img, p)
+
+
+def _kernel(kind, M=64, dx=-1e-5, with_gcf=True, **kw):
+    from gpuvmem_amd.gridding import GriddingKernel
+    return GriddingKernel.build(kind, M, M, dx, -dx, with_gcf=with_gcf, **kw)
+
+
+def test_ckernel_pillbox_and_gaussian_closed_forms():
+    k = _kernel("pillbox")
+    assert k.taps.shape == (1, 1) and k.taps[0, 0] == 1.0 and k.support_x == 0
+    assert np.all(k.gcf == 1.0)
+    k = _kernel("pillbox", m=5, n=5)
+    assert np.all(k.taps == 1.0) and k.support_x == k.support_y == 2
+    # Gaussian2D: exp(-(|x|/(w s))^a - (|y|/(w s))^a) inside |x|, |y| < w s
+    for w, a in ((3.0, 2.0), (2.5, 1.0)):
+        k = _kernel("gaussian", w=w, alpha=a)
+        d = np.abs(np.arange(7) - 3)
+        f = np.where(d < w, np.exp(-(d / w) ** a), 0.0)
+        ref = np.outer(f, f)
+        assert np.allclose(k.taps, ref, rtol=2e-6, atol=0)
+    # the default (w = 1) keeps the centre tap only
+    k = _kernel("gaussian")
+    assert k.taps[3, 3] == 1.0 and np.count_nonzero(k.taps) == 1
+    # its correction image: the Gaussian at pi x with width 2 M cells
+    M = 64
+    d = np.abs(np.arange(M) - M // 2)
+    g = np.exp(-(np.pi * d / (2 * M)) ** 2)
+    assert np.allclose(k.gcf, np.outer(g, g), rtol=2e-6)
+
+
+def test_ckernel_sinc_and_gaussian_sinc_closed_forms():
+    # Sinc2D::buildKernel hands (x, y) to sinc2D's (x, x0) slots: the tap
+    # is sinc(|x - y| / (w s)), a band along the diagonal
+    k = _kernel("sinc", with_gcf=False, w=3.0)
+    i, j = np.indices((7, 7))
+    ref = np.where(np.abs(j - i) < 3, np.sinc((j - i) / 3.0), 0.0)
+    assert np.allclose(k.taps, ref, rtol=2e-6, atol=1e-7)
+    # GaussianSinc2D (w = 2.52, w2 = 1.55, alpha = 2): nonzero for |x|, |y| <= 1 cell
+    k = _kernel("gaussian_sinc", with_gcf=False)
+    d = np.abs(np.arange(7) - 3).astype(float)
+    f = np.where(d < 1.55, np.exp(-(d / 2.52) ** 2) * np.sinc(d / 1.55), 0.0)
+    assert np.allclose(k.taps, np.outer(f, f), rtol=2e-6, atol=1e-7)
+    # neither defines a correction image in the reference
+    from gpuvmem_amd.gridding import GriddingKernel
+    for kind in ("sinc", "gaussian_sinc"):
+        with pytest.raises(ValueError):
+            GriddingKernel.build(kind, 64, 64, -1e-5, 1e-5)
+
+
+def test_ckernel_pswf_properties():
+    # a long 1-D profile: m = 201, w = 100 gives nu = |j - 100| / 100
+    k = _kernel("pswf", with_gcf=False, m=201, n=201, w=100.0)
+    row = k.taps[100].astype(np.float64)
+    nu = np.abs(np.arange(201) - 100) / 100.0
+    assert row[100] == 1.0 and np.allclose(row, row[::-1])
+    psi = row[101:200] / (1 - nu[101:200] ** 2)
+    # Schwab's function is normalised to 1 at nu = 0 and smooth across the
+    # break between its two rational pieces at nu = 0.75
+    assert abs(psi[0] - 1.0) < 1e-3
+    at = 74  # nu = 0.75
+    second = psi[at - 1] - 2 * psi[at] + psi[at + 1]
+    assert abs(second) < 5e-3 * psi[at]
+    assert np.all(np.diff(psi) < 0)
+    assert row[200] == 0.0 and 0 < row[199] < 1e-2
+    # the correction image is the reciprocal of the separable kernel with w = M
+    k = _kernel("pswf")
+    g = k.gcf.astype(np.float64)
+    assert g[32, 32] == 1.0
+    assert np.allclose(g, np.outer(g[:, 32], g[32, :]), rtol=1e-5)
+    assert np.all(np.diff(g[32, 32:]) > 0)
