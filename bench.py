@@ -333,6 +333,8 @@ def main() -> int:
     ap.add_argument("--precision-study", action="store_true",
                     help="instead of evals/s: each adjoint form against FP64 (use with "
                          "--workload 2: 2048^2, 4.35M vis)")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay captured HIP graphs of the value and gradient evaluations")
     ap.add_argument("--gridding", action="store_true",
                     help="time the device weighting and gridder and the gridded objective step")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -379,6 +381,7 @@ def main() -> int:
     ctx = Context(p, adjoint=args.adjoint)
     terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
     obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
+    obj.set_graphs(args.graphs)
     d_img = ctx.image(bench_image(p))
     xi = ctx.zeros(2 * p.M * p.N)
 
@@ -386,15 +389,25 @@ def main() -> int:
         obj.value(d_img)
         obj.gradient(d_img, xi, 1)
 
-    for _ in range(args.warmup):
+    # with graphs, the third evaluation of a kind is the first replay
+    for _ in range(max(args.warmup, 3) if args.graphs else args.warmup):
         step()
     ctx.sync()
     ctx.reset_timing()
+    replays0 = obj.graph_launches()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    replays = obj.graph_launches() - replays0
+    if args.graphs:
+        # the adjoint's launch timing is taken outside graphs: a few eager steps
+        obj.set_graphs(False)
+        ctx.reset_timing()
+        for _ in range(3):
+            step()
+        ctx.sync()
     adj_ms, adj_n = ctx.adjoint_time_ms()
     print(f"# {args.steps} steps in {elapsed:.3f} s; adjoint kernel {adj_ms:.3f} ms avg over "
           f"{adj_n} launches", file=sys.stderr, flush=True)
@@ -427,6 +440,8 @@ def main() -> int:
             "visibilities": nvt,
             "slices": nsl,
             "adjoint": args.adjoint,
+            "graphs": bool(args.graphs),
+            "graph_replays": replays,
         },
         "roofline": {
             "bound": "mfma",

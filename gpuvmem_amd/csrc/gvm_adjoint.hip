@@ -570,10 +570,16 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
   const int mode = ctx->adjoint_mode;
   const int nsl = (int)ctx->slices.size();
   if (ctx->split_mode != mode) {
+    GVM_REQUIRE(!ctx->capturing,
+                "gvm_dchi2: the workspace is sized by the first gvm_dchi2; run one before "
+                "capturing");
     gvm_status st = build_splits(ctx, mode, ntiles, n);
     if (st) return st;
+    ctx->epoch++;
   }
-  GVM_HIP_TRY(hipEventRecord(ctx->ev_a, ctx->stream));
+  // launch timing (not recorded inside graphs)
+  const bool timed = !ctx->capturing;
+  if (timed) GVM_HIP_TRY(hipEventRecord(ctx->ev_a, ctx->stream));
   if (mode == GVM_ADJOINT_DIRECT) {
     const double dxr = g.deltax * 3.14159265358979323846 / 180.0;
     const double dyr = g.deltay * 3.14159265358979323846 / 180.0;
@@ -600,12 +606,13 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
                          ctx->cmax, ctx->adj_part);
     GVM_CHECK_LAUNCH();
   }
-  GVM_HIP_TRY(hipEventRecord(ctx->ev_b, ctx->stream));
+  if (timed) GVM_HIP_TRY(hipEventRecord(ctx->ev_b, ctx->stream));
   hipLaunchKernelGGL(adjoint_epilogue_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      ctx->stream, ctx->adj_part, ctx->slice_p0, ctx->slice_tab, nsl, n, ctx->cmax,
                      mode == GVM_ADJOINT_DIRECT ? 0 : 1, ctx->noise, g.noise_cut, g.fg_scale,
                      ctx->gcf, normalize, g.threshold, flag_opt, I, result);
   GVM_CHECK_LAUNCH();
+  if (!timed) return GVM_OK;
   GVM_HIP_TRY(hipEventSynchronize(ctx->ev_b));
   float ms = 0.f;
   GVM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));

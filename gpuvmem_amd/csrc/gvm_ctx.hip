@@ -107,12 +107,24 @@ gvm_status finish_reductions_to(gvm_ctx* ctx, int nslots, int nblocks, double* o
   return GVM_OK;
 }
 
-gvm_status values_read(gvm_ctx* ctx, int n, double* out) {
+gvm_status values_fetch_enqueue(gvm_ctx* ctx, int n) {
   GVM_HIP_TRY(hipMemcpyAsync(ctx->hvalues, ctx->dvalues, sizeof(double) * n,
                              hipMemcpyDeviceToHost, ctx->stream));
+  return GVM_OK;
+}
+
+gvm_status values_host(gvm_ctx* ctx, int n, double* out) {
+  GVM_REQUIRE(!ctx->capturing, "values: cannot wait for the stream while capturing");
   GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
   for (int s = 0; s < n; s++) out[s] = ctx->hvalues[s];
   return GVM_OK;
+}
+
+gvm_status values_read(gvm_ctx* ctx, int n, double* out) {
+  GVM_REQUIRE(!ctx->capturing, "values: cannot wait for the stream while capturing");
+  gvm_status st = values_fetch_enqueue(ctx, n);
+  if (st) return st;
+  return values_host(ctx, n, out);
 }
 
 gvm_status finish_reductions(gvm_ctx* ctx, int nslots, int nblocks) {
@@ -183,10 +195,12 @@ gvm_status register_slice(gvm_ctx* ctx, Slice& s) {
   ctx->slices.push_back(s);
   ctx->split_mode = -1;
   ctx->have_residuals = false;
+  ctx->epoch++;
   return GVM_OK;
 }
 
 gvm_status read_scalars(gvm_ctx* ctx, int nslots, double* out) {
+  GVM_REQUIRE(!ctx->capturing, "cannot wait for the stream while capturing");
   GVM_HIP_TRY(hipMemcpyAsync(ctx->hscalars, ctx->dscalars, sizeof(double) * nslots,
                              hipMemcpyDeviceToHost, ctx->stream));
   GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -247,6 +261,7 @@ extern "C" gvm_status gvm_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 
 extern "C" gvm_status gvm_sync(gvm_ctx* ctx) {
   GVM_REQUIRE(ctx, "gvm_sync: null context");
+  GVM_REQUIRE(!ctx->capturing, "gvm_sync: cannot wait for the stream while capturing");
   GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return GVM_OK;
 }
@@ -424,6 +439,7 @@ extern "C" gvm_status gvm_set_adjoint(gvm_ctx* ctx, int mode) {
   GVM_REQUIRE(mode == GVM_ADJOINT_F32 || mode == GVM_ADJOINT_F16X3 ||
                   mode == GVM_ADJOINT_BF16X3 || mode == GVM_ADJOINT_DIRECT,
               "gvm_set_adjoint: unknown mode");
+  if (ctx->adjoint_mode != mode) ctx->epoch++;
   ctx->adjoint_mode = mode;
   return GVM_OK;
 }
@@ -560,6 +576,96 @@ extern "C" gvm_status gvm_values_read(gvm_ctx* ctx, int n, double* out) {
   GVM_REQUIRE(n >= 1 && n <= kValueSlots, "gvm_values_read: slot count out of range");
   GVM_HIP_TRY(hipSetDevice(ctx->device));
   return values_read(ctx, n, out);
+}
+
+extern "C" gvm_status gvm_values_fetch_enqueue(gvm_ctx* ctx, int n) {
+  GVM_REQUIRE(ctx, "gvm_values_fetch_enqueue: null context");
+  GVM_REQUIRE(n >= 1 && n <= kValueSlots, "gvm_values_fetch_enqueue: slot count out of range");
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  return values_fetch_enqueue(ctx, n);
+}
+
+extern "C" gvm_status gvm_values_host(gvm_ctx* ctx, int n, double* out) {
+  GVM_REQUIRE(ctx && out, "gvm_values_host: null argument");
+  GVM_REQUIRE(n >= 1 && n <= kValueSlots, "gvm_values_host: slot count out of range");
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  return values_host(ctx, n, out);
+}
+
+// ---- HIP graphs ----------------------------------------------------------
+
+struct gvm_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const gvm_ctx* ctx = nullptr;
+  long epoch = 0;
+};
+
+extern "C" gvm_status gvm_capture_begin(gvm_ctx* ctx) {
+  GVM_REQUIRE(ctx, "gvm_capture_begin: null context");
+  GVM_REQUIRE(!ctx->capturing, "gvm_capture_begin: already capturing");
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  GVM_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_capture_end(gvm_ctx* ctx, gvm_graph** out) {
+  GVM_REQUIRE(ctx && out, "gvm_capture_end: null argument");
+  GVM_REQUIRE(ctx->capturing, "gvm_capture_end: not capturing");
+  *out = nullptr;
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  ctx->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    set_error(std::string("gvm_capture_end: the capture failed (a captured call waited or "
+                          "failed): ") + hipGetErrorString(e));
+    return GVM_ERR_STATE;
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e2 != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    set_error(std::string("gvm_capture_end: hipGraphInstantiate: ") + hipGetErrorString(e2));
+    return GVM_ERR_HIP;
+  }
+  gvm_graph* gr = new gvm_graph;
+  gr->graph = g;
+  gr->exec = x;
+  gr->ctx = ctx;
+  gr->epoch = ctx->epoch;
+  *out = gr;
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_graph_launch(gvm_ctx* ctx, gvm_graph* graph) {
+  GVM_REQUIRE(ctx && graph, "gvm_graph_launch: null argument");
+  GVM_REQUIRE(graph->ctx == ctx, "gvm_graph_launch: graph of another context");
+  GVM_REQUIRE(!ctx->capturing, "gvm_graph_launch: capturing");
+  if (graph->epoch != ctx->epoch) {
+    set_error("gvm_graph_launch: the graph is stale (visibilities, adjoint form or GCF changed "
+              "since it was captured)");
+    return GVM_ERR_STATE;
+  }
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  GVM_HIP_TRY(hipGraphLaunch(graph->exec, ctx->stream));
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_graph_destroy(gvm_graph* graph) {
+  if (!graph) return GVM_OK;
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_epoch(const gvm_ctx* ctx, long* epoch) {
+  GVM_REQUIRE(ctx && epoch, "gvm_epoch: null argument");
+  *epoch = ctx->epoch;
+  return GVM_OK;
 }
 
 extern "C" gvm_status gvm_dchi2(gvm_ctx* ctx, const float* I_dev, float* result_dev,

@@ -531,8 +531,10 @@ extern "C" gvm_status gvm_add_visibilities_gridded_dev(gvm_ctx* ctx, const gvm_c
 
 extern "C" gvm_status gvm_set_gcf(gvm_ctx* ctx, const float* gcf_host) {
   GVM_REQUIRE(ctx, "gvm_set_gcf: null context");
+  GVM_REQUIRE(!ctx->capturing, "gvm_set_gcf: capturing");
   GVM_HIP_TRY(hipSetDevice(ctx->device));
   GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->epoch++;
   if (!gcf_host) {
     if (ctx->gcf) (void)hipFree(ctx->gcf);
     ctx->gcf = nullptr;

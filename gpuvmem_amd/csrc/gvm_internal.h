@@ -111,6 +111,10 @@ struct gvm_ctx {
   int adjoint_mode = GVM_ADJOINT_BF16X3;
   std::vector<gvm::Slice> slices;
   bool have_residuals = false;
+  // graph capture: true between gvm_capture_begin and gvm_capture_end; the
+  // epoch changes whenever recorded device pointers or launches go stale
+  bool capturing = false;
+  long epoch = 0;
   // timing of the adjoint GEMM
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   double adj_ms_total = 0.0;
@@ -134,6 +138,8 @@ gvm_status finish_reductions_to(gvm_ctx* ctx, int nslots, int nblocks, double* o
 gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot);
 // Copies dvalues[0, n) to the host (one synchronisation).
 gvm_status values_read(gvm_ctx* ctx, int n, double* out);
+gvm_status values_fetch_enqueue(gvm_ctx* ctx, int n);
+gvm_status values_host(gvm_ctx* ctx, int n, double* out);
 // Slices: frees one slice's device arrays; makes a filled slice the next
 // slice of ctx (grids, descriptor), freeing it on failure.
 void free_slice(Slice& s);

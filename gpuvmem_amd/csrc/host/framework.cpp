@@ -1,6 +1,8 @@
 // Fi base, ObjectiveFunction and the objective terms (Chi2, Entropy,
 // TotalSquaredVariation, Laplacian) over the libgvm_hip C ABI.
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <string>
 
 #include "gvm/factory.h"
@@ -50,6 +52,26 @@ Fi::~Fi() {
   gvm_free(device_DS);
 }
 
+namespace {
+int64_t key_bits(float f) {
+  int32_t b;
+  std::memcpy(&b, &f, sizeof b);
+  return b;
+}
+int64_t key_addr(const void* p) { return (int64_t)(uintptr_t)p; }
+constexpr size_t kGraphCacheEntries = 8;
+}  // namespace
+
+void Fi::graphKey(std::vector<int64_t>& key) const {
+  key.push_back(iteration > 0 ? 1 : 0);
+  key.push_back(key_bits(penalization_factor));
+  key.push_back(imageIndex);
+  key.push_back(imageToAdd);
+  key.push_back(normalize ? 1 : 0);
+  key.push_back(key_addr(device_S));
+  key.push_back(key_addr(device_DS));
+}
+
 void Fi::configure(int penalizatorIndex, int imageIndex, int imageToAdd, bool normalize) {
   const auto& g = globals();
   this->imageIndex = imageIndex;
@@ -74,7 +96,54 @@ void Fi::configure(int penalizatorIndex, int imageIndex, int imageToAdd, bool no
 // ---------------------------------------------------------------------------
 // ObjectiveFunction: include/classes/objectivefunction.cuh:6-103
 // ---------------------------------------------------------------------------
-ObjectiveFunction::~ObjectiveFunction() { gvm_free(dphi); }
+ObjectiveFunction::~ObjectiveFunction() {
+  clearGraphs();
+  gvm_free(dphi);
+}
+
+void ObjectiveFunction::clearGraphs() {
+  for (auto* c : {&value_graphs, &grad_graphs}) {
+    for (auto& e : *c) gvm_graph_destroy(e.graph);
+    c->clear();
+  }
+}
+
+void ObjectiveFunction::setGraphs(bool on) {
+  if (!on) clearGraphs();
+  graphs = on;
+}
+
+std::vector<int64_t> ObjectiveFunction::baseKey() const {
+  long epoch = 0;
+  check(gvm_epoch(gvm::ctx(), &epoch), "gvm_epoch");
+  const auto& g = globals();
+  return {key_addr(gvm::ctx()), epoch, g.flag_opt, g.M, g.N, g.image_count, key_addr(dphi),
+          (int64_t)fis.size()};
+}
+
+// The cache entry of `key` (created if new; the least recently used entry is
+// dropped beyond kGraphCacheEntries).
+ObjectiveFunction::GraphEntry* ObjectiveFunction::graphEntry(std::vector<GraphEntry>& cache,
+                                                             const std::vector<int64_t>& key) {
+  graph_clock++;
+  for (auto& e : cache)
+    if (e.key == key) {
+      e.used = graph_clock;
+      return &e;
+    }
+  if (cache.size() >= kGraphCacheEntries) {
+    size_t old = 0;
+    for (size_t i = 1; i < cache.size(); i++)
+      if (cache[i].used < cache[old].used) old = i;
+    gvm_graph_destroy(cache[old].graph);
+    cache.erase(cache.begin() + (long)old);
+  }
+  GraphEntry e;
+  e.key = key;
+  e.used = graph_clock;
+  cache.push_back(e);
+  return &cache.back();
+}
 
 void ObjectiveFunction::addFi(Fi* fi) {
   if (fi->getPenalizationFactor()) {
@@ -90,10 +159,45 @@ double ObjectiveFunction::calcFunctionD(float* p) {
   for (Fi* fi : fis) async = async && fi->hasAsyncValue();
   if (async) {
     // the same evaluations in the same stream order, one synchronisation
+    const int nv = (int)fis.size();
     std::vector<char> queued(fis.size());
-    for (size_t k = 0; k < fis.size(); k++) queued[k] = fis[k]->enqueueValue(p, (int)k) ? 1 : 0;
     std::vector<double> raw(fis.size(), 0.0);
-    check(gvm_values_read(gvm::ctx(), (int)fis.size(), raw.data()), "gvm_values_read");
+    GraphEntry* e = nullptr;
+    if (graphs) {
+      std::vector<int64_t> key = baseKey();
+      key.push_back(1);
+      key.push_back(key_addr(p));
+      for (Fi* fi : fis) fi->graphKey(key);
+      e = graphEntry(value_graphs, key);
+      if (!e->graph && !e->failed && e->seen >= 1) {
+        check(gvm_capture_begin(gvm::ctx()), "gvm_capture_begin");
+        bool ok = true;
+        try {
+          for (size_t k = 0; k < fis.size(); k++) queued[k] = fis[k]->enqueueValue(p, (int)k);
+          check(gvm_values_fetch_enqueue(gvm::ctx(), nv), "gvm_values_fetch_enqueue");
+        } catch (const gvm::Error&) {
+          ok = false;
+        }
+        gvm_graph* gr = nullptr;
+        if (gvm_capture_end(gvm::ctx(), &gr) == GVM_OK && ok) {
+          e->graph = gr;
+          e->queued = queued;
+        } else {
+          gvm_graph_destroy(gr);
+          e->failed = true;
+        }
+      }
+    }
+    if (e && e->graph) {
+      check(gvm_graph_launch(gvm::ctx(), e->graph), "gvm_graph_launch");
+      check(gvm_values_host(gvm::ctx(), nv, raw.data()), "gvm_values_host");
+      queued = e->queued;
+      graph_launches++;
+    } else {
+      for (size_t k = 0; k < fis.size(); k++) queued[k] = fis[k]->enqueueValue(p, (int)k) ? 1 : 0;
+      check(gvm_values_read(gvm::ctx(), nv, raw.data()), "gvm_values_read");
+      if (e) e->seen++;
+    }
     for (size_t k = 0; k < fis.size(); k++) {
       value += fis[k]->finishValue(queued[k] ? raw[k] : 0.0);
       fi_values[k] = fis[k]->get_fivalue();
@@ -109,8 +213,7 @@ double ObjectiveFunction::calcFunctionD(float* p) {
 
 float ObjectiveFunction::calcFunction(float* p) { return (float)calcFunctionD(p); }
 
-void ObjectiveFunction::calcGradient(float* p, float* xi, int iter) {
-  n_gcalls++;
+void ObjectiveFunction::gradientSequence(float* p, float* xi, int iter) {
   restartDPhi();
   for (Fi* fi : fis) {
     fi->setIteration(iter);
@@ -118,6 +221,43 @@ void ObjectiveFunction::calcGradient(float* p, float* xi, int iter) {
     fi->addToDphi(dphi);
   }
   copyDphiToXi(xi);
+}
+
+void ObjectiveFunction::calcGradient(float* p, float* xi, int iter) {
+  n_gcalls++;
+  GraphEntry* e = nullptr;
+  if (graphs) {
+    for (Fi* fi : fis) fi->setIteration(iter);
+    std::vector<int64_t> key = baseKey();
+    key.push_back(2);
+    key.push_back(key_addr(p));
+    key.push_back(key_addr(xi));
+    for (Fi* fi : fis) fi->graphKey(key);
+    e = graphEntry(grad_graphs, key);
+    if (!e->graph && !e->failed && e->seen >= 1) {
+      check(gvm_capture_begin(gvm::ctx()), "gvm_capture_begin");
+      bool ok = true;
+      try {
+        gradientSequence(p, xi, iter);
+      } catch (const gvm::Error&) {
+        ok = false;
+      }
+      gvm_graph* gr = nullptr;
+      if (gvm_capture_end(gvm::ctx(), &gr) == GVM_OK && ok) {
+        e->graph = gr;
+      } else {
+        gvm_graph_destroy(gr);
+        e->failed = true;
+      }
+    }
+    if (e->graph) {
+      check(gvm_graph_launch(gvm::ctx(), e->graph), "gvm_graph_launch");
+      graph_launches++;
+      return;
+    }
+  }
+  gradientSequence(p, xi, iter);
+  if (e) e->seen++;
 }
 
 void ObjectiveFunction::restartDPhi() {
@@ -180,6 +320,11 @@ double Chi2::calcFiD(float* p) {
 
 float Chi2::calcFi(float* p) { return (float)calcFiD(p); }
 
+void Chi2::graphKey(std::vector<int64_t>& key) const {
+  Fi::graphKey(key);
+  key.push_back(key_addr(result_dchi2));
+}
+
 bool Chi2::enqueueValue(float* p, int slot) {
   check(gvm_chi2_enqueue(gvm::ctx(), p, globals().flag_opt, normalize ? 1 : 0, slot),
         "gvm_chi2_enqueue");
@@ -229,6 +374,12 @@ double Entropy::calcFiD(float* p) {
   return (double)penalization_factor * v;
 }
 float Entropy::calcFi(float* p) { return (float)calcFiD(p); }
+void Entropy::graphKey(std::vector<int64_t>& key) const {
+  Fi::graphKey(key);
+  key.push_back(key_bits(prior_value));
+  key.push_back(key_bits(eta));
+}
+
 bool Entropy::enqueueValue(float* p, int slot) {
   if (!prior_active(iteration, penalization_factor)) return false;
   check(gvm_entropy_value_enqueue(gvm::ctx(), p, imageIndex, prior_value, eta, slot),

@@ -178,6 +178,24 @@ extern "C" gvm_status gvm_fw_set_flag(gvm_fw_objective* obj, int flag_opt) {
   return GVM_OK;
 }
 
+extern "C" gvm_status gvm_fw_set_graphs(gvm_fw_objective* obj, int on) {
+  if (!obj || !obj->of) {
+    g_fw_error = "gvm_fw_set_graphs: bad argument";
+    return GVM_ERR_ARG;
+  }
+  obj->of->setGraphs(on != 0);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_graph_launches(gvm_fw_objective* obj, long* n) {
+  if (!obj || !obj->of || !n) {
+    g_fw_error = "gvm_fw_graph_launches: bad argument";
+    return GVM_ERR_ARG;
+  }
+  *n = obj->of->getGraphLaunches();
+  return GVM_OK;
+}
+
 extern "C" int gvm_fw_registered(const char* kind, const char* name) {
   if (!kind || !name) return 0;
   const std::string k(kind), n(name);

@@ -53,6 +53,25 @@ class DeviceBuffer:
             pass
 
 
+class Graph:
+    """A HIP graph captured on a context (gvm_capture_begin / gvm_capture_end)."""
+
+    def __init__(self, ctx: "Context", handle: int):
+        self.ctx = ctx
+        self.h = handle
+
+    def launch(self) -> None:
+        native.check(self.ctx.lib.gvm_graph_launch(self.ctx.ctx, self.h), "gvm_graph_launch")
+
+    def status_of_launch(self) -> int:
+        return self.ctx.lib.gvm_graph_launch(self.ctx.ctx, self.h)
+
+    def destroy(self) -> None:
+        if self.h:
+            self.ctx.lib.gvm_graph_destroy(self.h)
+            self.h = None
+
+
 class Context:
     """A gvm_ctx with one slice per call to :meth:`add`."""
 
@@ -144,6 +163,28 @@ class Context:
 
     def sync(self) -> None:
         native.check(self.lib.gvm_sync(self.ctx), "gvm_sync")
+
+    # ---- HIP graphs of enqueued work ----
+    def capture_begin(self) -> None:
+        native.check(self.lib.gvm_capture_begin(self.ctx), "gvm_capture_begin")
+
+    def capture_end(self) -> "Graph":
+        g = C.c_void_p()
+        native.check(self.lib.gvm_capture_end(self.ctx, C.byref(g)), "gvm_capture_end")
+        return Graph(self, g.value)
+
+    def chi2_enqueue(self, img: DeviceBuffer, slot: int, flag_opt: int = 0,
+                     normalize: bool = False) -> None:
+        native.check(self.lib.gvm_chi2_enqueue(self.ctx, img.ptr, flag_opt, int(normalize), slot),
+                     "gvm_chi2_enqueue")
+
+    def values_fetch_enqueue(self, n: int) -> None:
+        native.check(self.lib.gvm_values_fetch_enqueue(self.ctx, n), "gvm_values_fetch_enqueue")
+
+    def values_host(self, n: int) -> list[float]:
+        out = (C.c_double * n)()
+        native.check(self.lib.gvm_values_host(self.ctx, n, out), "gvm_values_host")
+        return list(out)
 
     def adjoint_time_ms(self) -> tuple[float, int]:
         ms = C.c_double()
@@ -239,6 +280,15 @@ class Objective:
                                                       secs, iterations), "gvm_fw_optimize_timed")
         n = min(res.iterations, iterations)
         return res, [traj[k] for k in range(n)], [secs[k] for k in range(n)]
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay captured HIP graphs of repeated evaluations."""
+        native.check_fw(self.fw.gvm_fw_set_graphs(self.h, int(on)), "gvm_fw_set_graphs")
+
+    def graph_launches(self) -> int:
+        n = C.c_long()
+        native.check_fw(self.fw.gvm_fw_graph_launches(self.h, C.byref(n)), "gvm_fw_graph_launches")
+        return n.value
 
     def set_flag(self, flag_opt: int) -> None:
         """Optimizer::setFlag for the following calls (schedule flag)."""

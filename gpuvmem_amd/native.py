@@ -135,6 +135,13 @@ HIP_API = {
     "gvm_add_visibilities_gridded_dev": [_P, C.POINTER(CKernel), _F, _L, _P, _P, _P,
                                          C.POINTER(_L), _IP],
     "gvm_set_gcf": [_P, _P],
+    "gvm_values_fetch_enqueue": [_P, _I],
+    "gvm_values_host": [_P, _I, _DP],
+    "gvm_capture_begin": [_P],
+    "gvm_capture_end": [_P, C.POINTER(_P)],
+    "gvm_graph_launch": [_P, _P],
+    "gvm_graph_destroy": [_P],
+    "gvm_epoch": [_P, C.POINTER(_L)],
     "gvm_weighting_apply_dev": [_P, _I, _F, C.POINTER(UVTaper), _I, _P, _P, _P, _P],
     "gvm_setup_beam_noise_dev": [_I, _P, _P, _P, _P, _D, _DP, _DP, _DP, _DP, _DP, _DP],
     "gvm_setup_beam_from_moments": [_D, _D, _D, _D, _D, _DP, _DP, _DP, _DP, _DP, _DP],
@@ -205,6 +212,8 @@ FW_API = {
     "gvm_fw_optimize_timed": ([_P, C.c_char_p, _I, _D, _D, _I, _P, C.POINTER(FwResult), _DP, _DP,
                                _I], _I),
     "gvm_fw_set_flag": ([_P, _I], _I),
+    "gvm_fw_set_graphs": ([_P, _I], _I),
+    "gvm_fw_graph_launches": ([_P, C.POINTER(_L)], _I),
     "gvm_fw_last_error": ([], C.c_char_p),
     "gvm_fw_registered": ([C.c_char_p, C.c_char_p], _I),
 }

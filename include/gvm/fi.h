@@ -5,7 +5,9 @@
 // their line-search scalars.
 #pragma once
 
+#include <cstdint>
 #include <string>
+#include <vector>
 
 #include "gvm/framework.h"
 
@@ -28,6 +30,9 @@ class Fi {
   // returns the weighted one.
   virtual bool hasAsyncValue() const { return false; }
   virtual bool enqueueValue(float* p, int slot) { return false; }
+  // Appends every value that decides which kernels this term launches and
+  // with which scalar arguments (the key of a captured evaluation graph).
+  virtual void graphKey(std::vector<int64_t>& key) const;
   double finishValue(double raw) {
     set_fivalue(raw);
     return (double)penalization_factor * raw;

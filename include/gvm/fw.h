@@ -79,6 +79,11 @@ gvm_status gvm_fw_optimize_timed(gvm_fw_objective* obj, const char* optimizer,
  * optimiser with flags 0, 1, 2, 3 in turn on the same image. */
 gvm_status gvm_fw_set_flag(gvm_fw_objective* obj, int flag_opt);
 
+/* ObjectiveFunction::setGraphs: replay captured HIP graphs of repeated value
+ * and gradient evaluations (off by default; results are identical). */
+gvm_status gvm_fw_set_graphs(gvm_fw_objective* obj, int on);
+/* Graph replays so far (value and gradient evaluations served by a graph). */
+gvm_status gvm_fw_graph_launches(gvm_fw_objective* obj, long* n);
 const char* gvm_fw_last_error(void);
 
 /* 1 if `name` is registered with the factory of `kind` ("Fi", "Optimizer",

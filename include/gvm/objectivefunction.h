@@ -28,8 +28,31 @@ class ObjectiveFunction {
   std::vector<float> get_fi_values() { return this->fi_values; }
   long getFunctionCalls() const { return n_fcalls; }
   long getGradientCalls() const { return n_gcalls; }
+  // Replays captured HIP graphs of the value and gradient evaluations: an
+  // evaluation whose key (pointers, epoch, flags, each term's graphKey) has
+  // been run once is captured the next time and replayed after that.  Off by
+  // default.  Results are identical to the uncaptured evaluation.
+  void setGraphs(bool on);
+  bool getGraphs() const { return graphs; }
+  long getGraphLaunches() const { return graph_launches; }
 
  private:
+  struct GraphEntry {
+    std::vector<int64_t> key;
+    gvm_graph* graph = nullptr;
+    std::vector<char> queued;
+    int seen = 0;
+    bool failed = false;
+    long used = 0;
+  };
+  GraphEntry* graphEntry(std::vector<GraphEntry>& cache, const std::vector<int64_t>& key);
+  std::vector<int64_t> baseKey() const;
+  void clearGraphs();
+  void gradientSequence(float* p, float* xi, int iter);
+  bool graphs = false;
+  std::vector<GraphEntry> value_graphs, grad_graphs;
+  long graph_clock = 0, graph_launches = 0;
+
   std::vector<Fi*> fis;
   std::vector<float> fi_values;
   float* dphi = nullptr;

@@ -14,6 +14,7 @@ class Chi2 : public Fi {
   double calcFiD(float* p) override;
   bool hasAsyncValue() const override { return true; }
   bool enqueueValue(float* p, int slot) override;
+  void graphKey(std::vector<int64_t>& key) const override;
   void calcGi(float* p, float* xi) override;
   void restartDGi() override;
   void addToDphi(float* device_dphi) override;
@@ -35,6 +36,7 @@ class Entropy : public Fi {
   double calcFiD(float* p) override;
   bool hasAsyncValue() const override { return true; }
   bool enqueueValue(float* p, int slot) override;
+  void graphKey(std::vector<int64_t>& key) const override;
   void calcGi(float* p, float* xi) override;
   void restartDGi() override;
   void addToDphi(float* device_dphi) override;

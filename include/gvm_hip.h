@@ -234,6 +234,31 @@ gvm_status gvm_tsv_value_enqueue(gvm_ctx* ctx, const float* I_dev, int index, in
 gvm_status gvm_laplacian_value_enqueue(gvm_ctx* ctx, const float* I_dev, int index, int slot);
 /* Waits for the stream and copies value slots [0, n) to out. */
 gvm_status gvm_values_read(gvm_ctx* ctx, int n, double* out);
+/* gvm_values_read in two halves: enqueue the copy of slots [0, n) to the
+ * context's pinned host buffer, then wait for the stream and read it. */
+gvm_status gvm_values_fetch_enqueue(gvm_ctx* ctx, int n);
+gvm_status gvm_values_host(gvm_ctx* ctx, int n, double* out);
+
+/* ---- HIP graphs of enqueued work ----------------------------------------- */
+/* No counterpart in the reference, which launches every kernel from the host
+ * each time.  gvm_capture_begin starts capturing the context's stream
+ * (thread-local capture mode); the calls that follow only record their work;
+ * gvm_capture_end instantiates it as a graph that gvm_graph_launch replays on
+ * the stream.  Only calls that enqueue without waiting may be captured: the
+ * *_enqueue value forms, gvm_values_fetch_enqueue, gvm_dchi2 (after one
+ * uncaptured gvm_dchi2 has sized its workspace), the prior gradients and the
+ * vector operations; a call that waits fails the capture.  A graph holds the
+ * device pointers it was recorded with and reads the data behind them when it
+ * runs.  Adding visibilities, changing the adjoint form or the GCF moves the
+ * context to a new epoch; a graph from an older epoch is refused
+ * (GVM_ERR_STATE). */
+typedef struct gvm_graph gvm_graph;
+gvm_status gvm_capture_begin(gvm_ctx* ctx);
+gvm_status gvm_capture_end(gvm_ctx* ctx, gvm_graph** graph);
+gvm_status gvm_graph_launch(gvm_ctx* ctx, gvm_graph* graph);
+gvm_status gvm_graph_destroy(gvm_graph* graph);
+/* A counter that changes whenever graphs of the context become invalid. */
+gvm_status gvm_epoch(const gvm_ctx* ctx, long* epoch);
 
 /* Replaces dchi2() (src/functions.cu:4456-4558) as called by Chi2::calcGi
  * (src/chi2.cu:51-53): per slice the DChi2 adjoint (:3698-3791) at the
