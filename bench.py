@@ -333,6 +333,8 @@ def main() -> int:
     ap.add_argument("--precision-study", action="store_true",
                     help="instead of evals/s: each adjoint form against FP64 (use with "
                          "--workload 2: 2048^2, 4.35M vis)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the adjoint's range length by timing before the run")
     ap.add_argument("--graphs", action="store_true",
                     help="replay captured HIP graphs of the value and gradient evaluations")
     ap.add_argument("--gridding", action="store_true",
@@ -384,6 +386,11 @@ def main() -> int:
     obj.set_graphs(args.graphs)
     d_img = ctx.image(bench_image(p))
     xi = ctx.zeros(2 * p.M * p.N)
+    tune = None
+    if args.autotune:
+        obj.value(d_img)  # residuals for the timed adjoints
+        tune = ctx.autotune()
+        print(f"# autotune: {tune}", file=sys.stderr, flush=True)
 
     def step():
         obj.value(d_img)
@@ -442,6 +449,8 @@ def main() -> int:
             "adjoint": args.adjoint,
             "graphs": bool(args.graphs),
             "graph_replays": replays,
+            "split_vis": ctx.split_vis(),
+            "autotune": tune,
         },
         "roofline": {
             "bound": "mfma",

@@ -34,6 +34,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "gvm_internal.h"
 
@@ -476,7 +482,7 @@ namespace {
 
 // Splits the visibilities of every slice into ranges for the GEMM
 // workgroups.  Each range accumulates in FP32 MFMA accumulators and the
-// ranges are combined in FP64, so ranges are kept near kSplitVis
+// ranges are combined in FP64, so ranges are kept near ctx->split_vis
 // visibilities (about 10^3 accumulation steps: the FP32 rounding error grows
 // with the square root of the accumulation length), with at least enough
 // ranges to fill the GPU, within a workspace of at most kMaxWorkspace bytes
@@ -484,7 +490,7 @@ namespace {
 // the XCD remap with empty ranges.  The direct form takes one range per
 // slice.
 gvm_status build_splits(gvm_ctx* ctx, int mode, int ntiles, long n) {
-  constexpr long kSplitVis = 3072;
+  const long kSplitVis = ctx->split_vis;
   constexpr long kMaxWorkspace = 32L << 30;
   const int nsl = (int)ctx->slices.size();
   std::vector<int4> tab;
@@ -560,6 +566,15 @@ gvm_status build_splits(gvm_ctx* ctx, int mode, int ntiles, long n) {
 
 }  // namespace
 
+gvm_status rebuild_splits(gvm_ctx* ctx) {
+  const int N = ctx->g.N;
+  const int tiles_x = (N + kTile - 1) / kTile;
+  gvm_status st = build_splits(ctx, ctx->adjoint_mode, tiles_x * tiles_x, (long)N * N);
+  if (st) return st;
+  ctx->epoch++;
+  return GVM_OK;
+}
+
 gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, int normalize) {
   const gvm_geometry& g = ctx->g;
   const int N = g.N;
@@ -622,3 +637,164 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
 }
 
 }  // namespace gvm
+
+using namespace gvm;
+
+extern "C" gvm_status gvm_set_split_vis(gvm_ctx* ctx, long split_vis) {
+  GVM_REQUIRE(ctx, "gvm_set_split_vis: null context");
+  GVM_REQUIRE(split_vis >= 256 && split_vis <= 16384,
+              "gvm_set_split_vis: the range length must be in [256, 16384]");
+  GVM_REQUIRE(!ctx->capturing, "gvm_set_split_vis: capturing");
+  if (ctx->split_vis != split_vis) {
+    ctx->split_vis = split_vis;
+    ctx->split_mode = -1;  // rebuilt by the next gvm_dchi2
+    ctx->epoch++;
+  }
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_get_split_vis(const gvm_ctx* ctx, long* split_vis) {
+  GVM_REQUIRE(ctx && split_vis, "gvm_get_split_vis: null argument");
+  *split_vis = ctx->split_vis;
+  return GVM_OK;
+}
+
+namespace {
+
+std::mutex g_tune_mutex;
+std::map<std::string, long> g_tune_cache;
+
+std::string tune_key(const gvm_ctx* ctx) {
+  long nv = 0;
+  for (const auto& s : ctx->slices) nv += s.nvis;
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "adjoint M=%d nvis=%ld slices=%d mode=%d cu=%d", ctx->g.M, nv,
+                (int)ctx->slices.size(), ctx->adjoint_mode, ctx->cu_count);
+  return buf;
+}
+
+// The cached range length of `key` (process table, then the GVM_TUNE_CACHE
+// file: lines "<key> = <value>", the last one for a key wins), or 0.
+long tune_lookup(const std::string& key) {
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  auto it = g_tune_cache.find(key);
+  if (it != g_tune_cache.end()) return it->second;
+  const char* path = std::getenv("GVM_TUNE_CACHE");
+  if (!path || !*path) return 0;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return 0;
+  long found = 0;
+  char line[256];
+  while (std::fgets(line, sizeof line, f)) {
+    std::string l(line);
+    const size_t eq = l.rfind(" = ");
+    if (eq == std::string::npos || l.compare(0, eq, key) != 0 || eq != key.size()) continue;
+    const long v = std::strtol(l.c_str() + eq + 3, nullptr, 10);
+    if (v >= 256 && v <= 16384) found = v;
+  }
+  std::fclose(f);
+  if (found) g_tune_cache[key] = found;
+  return found;
+}
+
+void tune_store(const std::string& key, long value) {
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  g_tune_cache[key] = value;
+  const char* path = std::getenv("GVM_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%s = %ld\n", key.c_str(), value);
+  std::fclose(f);
+}
+
+}  // namespace
+
+extern "C" gvm_status gvm_autotune_adjoint(gvm_ctx* ctx, const long* candidates, int ncand,
+                                           long* chosen, double* ms_out, int* cached) {
+  static const long kDefault[] = {1536, 2048, 3072, 4096, 6144, 8192};
+  GVM_REQUIRE(ctx && chosen, "gvm_autotune_adjoint: null argument");
+  GVM_REQUIRE(!ctx->capturing, "gvm_autotune_adjoint: capturing");
+  GVM_REQUIRE(ctx->adjoint_mode != GVM_ADJOINT_DIRECT,
+              "gvm_autotune_adjoint: the direct form has no range length");
+  if (!candidates) {
+    candidates = kDefault;
+    ncand = (int)(sizeof kDefault / sizeof kDefault[0]);
+  }
+  GVM_REQUIRE(ncand >= 1 && ncand <= 64, "gvm_autotune_adjoint: 1 to 64 candidates");
+  for (int c = 0; c < ncand; c++)
+    GVM_REQUIRE(candidates[c] >= 256 && candidates[c] <= 16384,
+                "gvm_autotune_adjoint: candidates must be in [256, 16384]");
+  if (!ctx->have_residuals) {
+    set_error("gvm_autotune_adjoint: no residuals (call gvm_chi2 first)");
+    return GVM_ERR_STATE;
+  }
+  if (cached) *cached = 0;
+  const std::string key = tune_key(ctx);
+  const long hit = tune_lookup(key);
+  if (hit) {
+    if (cached) *cached = 1;
+    *chosen = hit;
+    return gvm_set_split_vis(ctx, hit);
+  }
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  const long n2 = 2L * ctx->g.M * ctx->g.N;
+  float *img = nullptr, *res = nullptr;
+  hipEvent_t ea = nullptr, eb = nullptr;
+  auto cleanup = [&]() {
+    if (img) (void)hipFree(img);
+    if (res) (void)hipFree(res);
+    if (ea) (void)hipEventDestroy(ea);
+    if (eb) (void)hipEventDestroy(eb);
+  };
+  if (hipMalloc((void**)&img, sizeof(float) * n2) != hipSuccess ||
+      hipMalloc((void**)&res, sizeof(float) * n2) != hipSuccess ||
+      hipMemset(img, 0, sizeof(float) * n2) != hipSuccess ||
+      hipMemset(res, 0, sizeof(float) * n2) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess ||  // the context's stream does not wait for the null stream
+      hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess) {
+    cleanup();
+    set_error("gvm_autotune_adjoint: scratch allocation failed");
+    return GVM_ERR_OOM;
+  }
+  const double save_ms = ctx->adj_ms_total;
+  const long save_n = ctx->adj_count;
+  long best = candidates[0];
+  double best_ms = 1e300;
+  gvm_status st = GVM_OK;
+  for (int c = 0; c < ncand && st == GVM_OK; c++) {
+    ctx->split_vis = candidates[c];
+    if ((st = rebuild_splits(ctx))) break;
+    // one launch to warm up, then three timed (range table, kernels and epilogue)
+    if ((st = adjoint(ctx, img, res, 0, 0))) break;
+    if (hipEventRecord(ea, ctx->stream) != hipSuccess) { st = GVM_ERR_HIP; break; }
+    for (int r = 0; r < 3 && st == GVM_OK; r++) st = adjoint(ctx, img, res, 0, 0);
+    if (st) break;
+    float ms = 0.f;
+    if (hipEventRecord(eb, ctx->stream) != hipSuccess || hipEventSynchronize(eb) != hipSuccess ||
+        hipEventElapsedTime(&ms, ea, eb) != hipSuccess) {
+      st = GVM_ERR_HIP;
+      break;
+    }
+    const double avg = ms / 3.0;
+    if (ms_out) ms_out[c] = avg;
+    if (avg < best_ms) {
+      best_ms = avg;
+      best = candidates[c];
+    }
+  }
+  ctx->adj_ms_total = save_ms;
+  ctx->adj_count = save_n;
+  if (st) {
+    cleanup();
+    if (st == GVM_ERR_HIP) set_error("gvm_autotune_adjoint: timing failed");
+    return st;
+  }
+  cleanup();
+  ctx->split_vis = best;
+  ctx->split_mode = -1;
+  ctx->epoch++;
+  tune_store(key, best);
+  *chosen = best;
+  return GVM_OK;
+}

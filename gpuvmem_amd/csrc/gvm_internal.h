@@ -109,6 +109,7 @@ struct gvm_ctx {
   int nsplit_total = 0;
   int split_mode = -1;          // adjoint mode the table was built for (-1: stale)
   int adjoint_mode = GVM_ADJOINT_BF16X3;
+  long split_vis = 3072;        // target visibilities per adjoint range (K tile)
   std::vector<gvm::Slice> slices;
   bool have_residuals = false;
   // graph capture: true between gvm_capture_begin and gvm_capture_end; the
@@ -128,6 +129,8 @@ gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize,
                         double* out);
 gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt,
                    int normalize);
+// Rebuilds the adjoint's range table and workspace for the current form.
+gvm_status rebuild_splits(gvm_ctx* ctx);
 // Block-partial reduction helpers: reduce `nblocks` partials of slot s into
 // dscalars[s] (device), in a fixed order.
 gvm_status finish_reductions(gvm_ctx* ctx, int nslots, int nblocks);

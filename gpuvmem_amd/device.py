@@ -186,6 +186,31 @@ class Context:
         native.check(self.lib.gvm_values_host(self.ctx, n, out), "gvm_values_host")
         return list(out)
 
+    def set_split_vis(self, n: int) -> None:
+        native.check(self.lib.gvm_set_split_vis(self.ctx, n), "gvm_set_split_vis")
+
+    def split_vis(self) -> int:
+        v = C.c_long()
+        native.check(self.lib.gvm_get_split_vis(self.ctx, C.byref(v)), "gvm_get_split_vis")
+        return v.value
+
+    def autotune(self, candidates=None) -> dict:
+        """gvm_autotune_adjoint: picks the adjoint's range length by timing
+        (a chi2 must have run).  Returns the choice, the candidates' times and
+        whether the shape was already tuned."""
+        if candidates is None:
+            cand, nc = None, 6
+        else:
+            cand, nc = (C.c_long * len(candidates))(*candidates), len(candidates)
+        ms = (C.c_double * nc)()
+        chosen, cached = C.c_long(), C.c_int()
+        native.check(self.lib.gvm_autotune_adjoint(self.ctx, cand, nc if cand else 0,
+                                                   C.byref(chosen), ms, C.byref(cached)),
+                     "gvm_autotune_adjoint")
+        names = list(candidates) if candidates else [1536, 2048, 3072, 4096, 6144, 8192]
+        return {"split_vis": chosen.value, "cached": bool(cached.value),
+                "ms": None if cached.value else dict(zip(names, list(ms)))}
+
     def adjoint_time_ms(self) -> tuple[float, int]:
         ms = C.c_double()
         cnt = C.c_long()

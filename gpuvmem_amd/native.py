@@ -142,6 +142,9 @@ HIP_API = {
     "gvm_graph_launch": [_P, _P],
     "gvm_graph_destroy": [_P],
     "gvm_epoch": [_P, C.POINTER(_L)],
+    "gvm_set_split_vis": [_P, _L],
+    "gvm_get_split_vis": [_P, C.POINTER(_L)],
+    "gvm_autotune_adjoint": [_P, _P, _I, C.POINTER(_L), _DP, _IP],
     "gvm_weighting_apply_dev": [_P, _I, _F, C.POINTER(UVTaper), _I, _P, _P, _P, _P],
     "gvm_setup_beam_noise_dev": [_I, _P, _P, _P, _P, _D, _DP, _DP, _DP, _DP, _DP, _DP],
     "gvm_setup_beam_from_moments": [_D, _D, _D, _D, _D, _DP, _DP, _DP, _DP, _DP, _DP],

@@ -394,6 +394,24 @@ gvm_status gvm_device_info(char* name, int name_len, int* cu_count,
 gvm_status gvm_timing_reset(gvm_ctx* ctx);
 gvm_status gvm_timing_adjoint_ms(gvm_ctx* ctx, double* avg_ms, long* count);
 
+/* ---- adjoint K tile: the visibility-range length ------------------------ */
+/* No counterpart in the reference.  Each workgroup of the split-precision
+ * adjoint accumulates one 256 x 256 output tile over a range of about
+ * `split_vis` visibilities in FP32 (the ranges are summed in FP64): shorter
+ * ranges keep the FP32 accumulation short, longer ones write fewer partial
+ * tiles.  Default 3072; allowed 256 .. 16384. */
+gvm_status gvm_set_split_vis(gvm_ctx* ctx, long split_vis);
+gvm_status gvm_get_split_vis(const gvm_ctx* ctx, long* split_vis);
+/* Autotuner: times the adjoint of the current residuals (a gvm_chi2 must
+ * have run) for each candidate range length (NULL: 1536, 2048, 3072, 4096,
+ * 6144, 8192), with HIP events, and keeps the fastest.  The choice is cached
+ * per shape (M, total visibilities, slices, adjoint form, CU count) for the
+ * process, and in the text file named by the GVM_TUNE_CACHE environment
+ * variable when it is set; a cached shape is not timed again (*cached = 1).
+ * ms_out (optional, ncand values) receives the timings. */
+gvm_status gvm_autotune_adjoint(gvm_ctx* ctx, const long* candidates, int ncand, long* chosen,
+                                double* ms_out, int* cached);
+
 #ifdef __cplusplus
 }
 #endif
