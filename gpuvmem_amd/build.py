@@ -23,7 +23,7 @@ LIB = PKG / "lib"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["gvm_ctx.hip", "gvm_forward.hip", "gvm_adjoint.hip", "gvm_ops.hip", "gvm_setup.hip",
-               "gvm_grid.hip"]
+               "gvm_grid.hip", "gvm_nn.hip"]
 HOST_SOURCES = ["framework.cpp", "optimizers.cpp", "setup.cpp", "fw_capi.cpp", "simdata.cpp",
                 "fitsio.cpp", "ckernels.cpp"]
 

@@ -142,6 +142,16 @@ HIP_API = {
     "gvm_graph_launch": [_P, _P],
     "gvm_graph_destroy": [_P],
     "gvm_epoch": [_P, C.POINTER(_L)],
+    "gvm_nn_conv_fwd": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I],
+    "gvm_nn_conv_bwd": [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I],
+    "gvm_nn_pool_fwd": [_P, _P, _P, _I, _I, _I, _I],
+    "gvm_nn_pool_bwd": [_P, _P, _P, _I, _I, _I, _I],
+    "gvm_nn_up_fwd": [_P, _P, _P, _I, _I, _I, _I, _I, _I],
+    "gvm_nn_up_bwd": [_P, _P, _P, _I, _I, _I, _I, _I, _I],
+    "gvm_nn_copy_channels": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I],
+    "gvm_nn_add": [_P, _P, _P, _L],
+    "gvm_nn_mse": [_P, _P, _P, _P, _L, _I],
+    "gvm_nn_adam": [_P, _P, _P, _P, _P, _L, _F, _F, _F, _F, _I],
     "gvm_set_split_vis": [_P, _L],
     "gvm_get_split_vis": [_P, C.POINTER(_L)],
     "gvm_autotune_adjoint": [_P, _P, _I, C.POINTER(_L), _DP, _IP],
