@@ -179,6 +179,52 @@ def bench_gridding(p, steps: int, warmup: int, threads: int, adjoint: str) -> di
     return out
 
 
+def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, levels: int,
+                     base: int) -> dict:
+    """The learned warm start at the 256^2 / 36,000-visibility scale: training
+    pairs from the plain solver (CG-FRPRMN to its tolerance), U-Net training on
+    the device kernels, then the plain solver from the default image and from
+    the prediction on held-out observations."""
+    import numpy as np
+    from gpuvmem_amd import warmstart as W
+    from gpuvmem_amd.device import Context
+
+    t0 = time.perf_counter()
+    chunks = []
+    for c0 in range(0, n_train, 16):
+        chunks.append(W.make_pairs(range(1000 + c0, 1000 + min(n_train, c0 + 16)),
+                                   iterations=iterations))
+        print(f"# pairs {c0 + len(chunks[-1].dirty)} ({time.perf_counter() - t0:.1f} s)",
+              file=sys.stderr, flush=True)
+    pairs = W.Pairs(np.concatenate([c.dirty for c in chunks]),
+                    np.concatenate([c.model for c in chunks]))
+    t_pairs = time.perf_counter() - t0
+    p = W.observation(1000)
+    ctx = Context(p, add_data=False)
+    ws = W.WarmStart(ctx, levels=levels, base=base, batch=8, npix=p.M)
+    t_train = ws.train(pairs, steps=steps)
+    print(f"# trained in {t_train:.1f} s, loss {ws.history[-1][1]:.3e}", file=sys.stderr, flush=True)
+    r = W.evaluate(ws, range(5000, 5000 + n_test), iterations=iterations)
+    rows = r.pop("rows")
+    out = {
+        "metric": "learned warm start: time to the objective both starts reach, default / warm",
+        "value": r["median_time_ratio"], "unit": "x (median over held-out observations)",
+        "higher_is_better": True, "n_gpus": 1, "data": "synthetic (seeded generator)",
+        "config": {"workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, chi2+entropy, "
+                               f"CG-FRPRMN to its tolerance (at most {iterations} iterations)",
+                   "unet": {"levels": levels, "base_channels": base, "parameters": ws.net.nparams,
+                            "weights": "fp32", "batch": 8},
+                   "train_pairs": n_train, "adam_steps": steps, "held_out": n_test},
+        "pairs_seconds": t_pairs, "train_seconds": t_train,
+        "train_loss": ws.history, **r,
+        "rows": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in row.items()}
+                 for row in rows],
+    }
+    ws.close()
+    ctx.close()
+    return out
+
+
 def profiled_traffic(mode: str, nvis: int):
     """HBM bytes per launch of the adjoint GEMM from the committed rocprofv3
     counter summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
@@ -337,6 +383,8 @@ def main() -> int:
                     help="pick the adjoint's range length by timing before the run")
     ap.add_argument("--graphs", action="store_true",
                     help="replay captured HIP graphs of the value and gradient evaluations")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="train the U-Net warm start and measure its gain (256^2 observations)")
     ap.add_argument("--gridding", action="store_true",
                     help="time the device weighting and gridder and the gridded objective step")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -350,6 +398,10 @@ def main() -> int:
     from gpuvmem_amd.device import Context, Objective, fw_config
     from gpuvmem_amd.problem import make_mfs_problem, make_problem
 
+    if args.warm_start:
+        r = bench_warm_start(n_train=96, n_test=12, steps=400, iterations=500, levels=4, base=16)
+        print(json.dumps(r), flush=True)
+        return 0
     t0 = time.perf_counter()
     p = make_mfs_problem() if args.workload == 3 else make_problem(args.workload)
     print(f"# workload {args.workload}: {p.M}x{p.N}, {len(p.all_slices())} slices, "

@@ -242,3 +242,180 @@ class UNet:
     def close(self) -> None:
         for b in [self.params, self.grads, self.m, self.v]:
             b.free()
+
+
+# ---------------------------------------------------------------------------
+# Training pairs from the plain solver, training, and the warm start itself
+# ---------------------------------------------------------------------------
+
+# the objective of the 256^2 workload: chi2 + Entropy (lambda 0.01, G = 0.001)
+TERMS = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001)]
+LAMBDAS = (0.01,)
+
+
+def observation(seed: int, **overrides):
+    """A 256^2 / 36,000-visibility observation of the workload's array with
+    the sky and the noise drawn from `seed` (the array layout stays fixed)."""
+    from .problem import make_problem
+
+    kw = dict(seed_sky=seed, seed_noise=seed + 7919)
+    kw.update(overrides)
+    return make_problem(0, **kw)
+
+
+def dirty_image(ctx: Context, p) -> np.ndarray:
+    """-d chi2 / d I at the default initial image (image 0), float32 (N, M)."""
+    d = ctx.image(p.initial_image())
+    ctx.chi2(d)
+    r = ctx.zeros(2 * p.M * p.N)
+    ctx.dchi2(d, r)
+    ctx.sync()
+    g = r.download(np.float32, (2, p.N, p.M))[0]
+    d.free()
+    r.free()
+    return -g
+
+
+def solve(p, init: Optional[np.ndarray] = None, iterations: int = 400,
+          optimizer: str = "CG-FRPRMN", graphs: bool = True):
+    """The plain MEM optimisation (CG-FRPRMN, the workload's optimiser, or
+    CG-LBFGS with memory 10), run to its own tolerance or `iterations`, from
+    the default image or `init` (image 0).  Returns (final image 0,
+    trajectory of objective values with the starting value first, seconds to
+    each entry, result)."""
+    ctx = Context(p)
+    obj = Objective(ctx, fw_config(TERMS, lambdas=LAMBDAS))
+    obj.set_graphs(graphs)
+    img = p.initial_image()
+    if init is not None:
+        img[0] = np.maximum(init, p.minpix).astype(np.float32)
+    d = ctx.image(img)
+    f0 = obj.value(d)
+    res, traj, secs = obj.optimize_timed(d, optimizer=optimizer, iterations=iterations,
+                                         lbfgs_k=10 if optimizer == "CG-LBFGS" else 0)
+    out = d.download(np.float32, (2, p.N, p.M))[0]
+    obj.close()
+    ctx.close()
+    return out, [f0] + list(traj), [0.0] + list(secs), res
+
+
+@dataclass
+class Pairs:
+    dirty: np.ndarray  # (n, N, M)
+    model: np.ndarray  # (n, N, M)
+    seeds: list = field(default_factory=list)
+    seconds: float = 0.0
+
+
+def make_pairs(seeds, iterations: int = 400, optimizer: str = "CG-FRPRMN") -> Pairs:
+    """Training pairs (dirty image, MEM solution) from the plain solver."""
+    t0 = time.perf_counter()
+    dirty, model = [], []
+    for s in seeds:
+        p = observation(s)
+        ctx = Context(p)
+        dirty.append(dirty_image(ctx, p))
+        ctx.close()
+        model.append(solve(p, iterations=iterations, optimizer=optimizer)[0])
+    return Pairs(np.stack(dirty), np.stack(model), list(seeds), time.perf_counter() - t0)
+
+
+class WarmStart:
+    """A trained U-Net and its input/output scaling."""
+
+    def __init__(self, ctx: Context, levels: int = 4, base: int = 16, batch: int = 8,
+                 npix: int = 256, seed: int = 0):
+        self.net = UNet(ctx, levels=levels, base=base, batch=batch, H=npix, W=npix, seed=seed)
+        self.ctx = ctx
+        self.calib = 1.0
+        self.history: list = []
+
+    def _scale(self, dirty):
+        return np.maximum(np.abs(dirty).reshape(len(dirty), -1).max(axis=1), 1e-30)
+
+    def train(self, pairs: Pairs, steps: int = 400, lr: float = 1e-3, seed: int = 0,
+              log_every: int = 50) -> float:
+        """Adam on the MSE of the scaled model; returns the seconds taken."""
+        n = len(pairs.dirty)
+        s = self._scale(pairs.dirty)
+        ms = np.maximum(np.abs(pairs.model).reshape(n, -1).max(axis=1), 1e-30)
+        self.calib = float(np.median(ms / s))
+        x = (pairs.dirty / s[:, None, None]).astype(np.float32)
+        y = (pairs.model / (s * self.calib)[:, None, None]).astype(np.float32)
+        B, H, W = self.net.B, self.net.H, self.net.W
+        target = self.ctx.zeros(B * H * W)
+        rng = np.random.default_rng(seed)
+        t0 = time.perf_counter()
+        for step in range(steps):
+            idx = rng.choice(n, B, replace=n < B)
+            self.net.set_input(x[idx][:, None])
+            target.upload(y[idx][:, None])
+            self.net.forward()
+            loss = self.net.loss(target)
+            self.net.backward()
+            self.net.adam(lr=lr)
+            if step % log_every == 0 or step == steps - 1:
+                self.history.append((step, loss))
+        self.ctx.sync()
+        target.free()
+        return time.perf_counter() - t0
+
+    def predict(self, dirty: np.ndarray) -> np.ndarray:
+        """Initial model images for a stack of dirty images (n, N, M)."""
+        B = self.net.B
+        out = []
+        s = self._scale(dirty)
+        x = (dirty / s[:, None, None]).astype(np.float32)
+        for k in range(0, len(x), B):
+            xb = np.zeros((B, 1, self.net.H, self.net.W), np.float32)
+            xb[: len(x[k:k + B]), 0] = x[k:k + B]
+            self.net.set_input(xb)
+            self.net.forward()
+            y = self.net.output()[:, 0]
+            out.extend(y[: len(x[k:k + B])])
+        return np.stack(out) * (s * self.calib)[:, None, None]
+
+    def close(self) -> None:
+        self.net.close()
+
+
+def time_to(traj, secs, target: float):
+    """(iterations, seconds) to the first trajectory entry <= target (entry 0
+    is the start), or None."""
+    for k, (f, t) in enumerate(zip(traj, secs)):
+        if f <= target:
+            return k, t
+    return None
+
+
+def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-FRPRMN") -> dict:
+    """L-BFGS from the default image and from the warm start on held-out
+    observations: iterations and seconds (dirty image and inference
+    included for the warm start) to the objective value both reach."""
+    rows = []
+    for s in seeds:
+        p = observation(s)
+        ctx = Context(p)
+        t0 = time.perf_counter()
+        dirty = dirty_image(ctx, p)
+        init = ws.predict(dirty[None])[0]
+        t_pre = time.perf_counter() - t0
+        ctx.close()
+        _, tr_d, sc_d, _ = solve(p, iterations=iterations, optimizer=optimizer)
+        img_w, tr_w, sc_w, _ = solve(p, init=init, iterations=iterations, optimizer=optimizer)
+        target = max(tr_d[-1], tr_w[-1])
+        a, b = time_to(tr_d, sc_d, target), time_to(tr_w, sc_w, target)
+        rows.append(dict(seed=s, f_default=tr_d[-1], f_warm=tr_w[-1], f0_default=tr_d[0],
+                         f0_warm=tr_w[0],
+                         iters_default=a[0], seconds_default=a[1],
+                         iters_warm=b[0], seconds_warm=b[1] + t_pre, warm_overhead_s=t_pre,
+                         iters_run_default=len(tr_d), iters_run_warm=len(tr_w)))
+    it_d = np.array([r["iters_default"] for r in rows], float)
+    it_w = np.maximum(np.array([r["iters_warm"] for r in rows], float), 1.0)
+    t_d = np.array([r["seconds_default"] for r in rows])
+    t_w = np.array([r["seconds_warm"] for r in rows])
+    return dict(rows=rows, median_iter_ratio=float(np.median(it_d / it_w)),
+                median_time_ratio=float(np.median(t_d / t_w)),
+                mean_iters_default=float(it_d.mean()), mean_iters_warm=float(it_w.mean()),
+                warm_better_or_equal_final=int(sum(r["f_warm"] <= r["f_default"] * (1 + 1e-6)
+                                                   for r in rows)))
