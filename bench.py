@@ -204,7 +204,7 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
     ws = W.WarmStart(ctx, levels=levels, base=base, batch=8, npix=p.M)
     t_train = ws.train(pairs, steps=steps)
     print(f"# trained in {t_train:.1f} s, loss {ws.history[-1][1]:.3e}", file=sys.stderr, flush=True)
-    r = W.evaluate(ws, range(5000, 5000 + n_test), iterations=iterations)
+    r = W.evaluate(ws, range(5000, 5000 + n_test), iterations=iterations, plain=True)
     rows = r.pop("rows")
     out = {
         "metric": "learned warm start: time to the objective both starts reach, default / warm",
@@ -213,7 +213,8 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
         "config": {"workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, chi2+entropy, "
                                f"CG-FRPRMN to its tolerance (at most {iterations} iterations)",
                    "unet": {"levels": levels, "base_channels": base, "parameters": ws.net.nparams,
-                            "weights": "fp32", "batch": 8},
+                            "weights": "bf16 (fp32 master, fp32 accumulation)" if ws.net.bf16
+                            else "fp32", "batch": 8},
                    "train_pairs": n_train, "adam_steps": steps, "held_out": n_test},
         "pairs_seconds": t_pairs, "train_seconds": t_train,
         "train_loss": ws.history, **r,

@@ -21,14 +21,20 @@ constexpr int kT = 256;
 
 int nblocks(long n) { return (int)std::max(1L, std::min<long>((n + kT - 1) / kT, 1L << 20)); }
 
+// A weight as float: FP32 storage, or BF16 (the upper half of an FP32).
+__device__ __forceinline__ float wload(const float* w, long i) { return w[i]; }
+__device__ __forceinline__ float wload(const unsigned short* w, long i) {
+  return __uint_as_float((unsigned int)w[i] << 16);
+}
+
 __device__ __forceinline__ float relu_grad(const float* out, const float* dout, long i, int relu) {
   const float g = dout[i];
   return relu ? (out[i] > 0.0f ? g : 0.0f) : g;
 }
 
-template <int K>
+template <int K, class WT>
 __global__ __launch_bounds__(kT) void conv_fwd_kernel(const float* __restrict__ in,
-                                                      const float* __restrict__ w,
+                                                      const WT* __restrict__ w,
                                                       const float* __restrict__ bias,
                                                       float* __restrict__ out, int B, int Ci,
                                                       int Co, int H, int W, int relu) {
@@ -41,11 +47,11 @@ __global__ __launch_bounds__(kT) void conv_fwd_kernel(const float* __restrict__ 
     const int co = (int)((o / HW) % Co);
     const int b = (int)(o / (HW * Co));
     const float* __restrict__ src = in + (long)b * Ci * HW;
-    const float* __restrict__ wc = w + (long)co * Ci * K * K;
+    const WT* __restrict__ wc = w + (long)co * Ci * K * K;
     float acc = bias[co];
     for (int ci = 0; ci < Ci; ci++) {
       const float* __restrict__ s = src + (long)ci * HW;
-      const float* __restrict__ wk = wc + ci * K * K;
+      const WT* __restrict__ wk = wc + ci * K * K;
 #pragma unroll
       for (int ky = 0; ky < K; ky++) {
         const int yy = y + ky - P;
@@ -54,7 +60,7 @@ __global__ __launch_bounds__(kT) void conv_fwd_kernel(const float* __restrict__ 
         for (int kx = 0; kx < K; kx++) {
           const int xx = x + kx - P;
           if (xx < 0 || xx >= W) continue;
-          acc = fmaf(wk[ky * K + kx], s[(long)yy * W + xx], acc);
+          acc = fmaf(wload(wk, ky * K + kx), s[(long)yy * W + xx], acc);
         }
       }
     }
@@ -63,8 +69,8 @@ __global__ __launch_bounds__(kT) void conv_fwd_kernel(const float* __restrict__ 
 }
 
 // din[b][ci][y][x] = sum_co sum_k w[co][ci][k] g[b][co][y - ky + P][x - kx + P]
-template <int K>
-__global__ __launch_bounds__(kT) void conv_bwd_data_kernel(const float* __restrict__ w,
+template <int K, class WT>
+__global__ __launch_bounds__(kT) void conv_bwd_data_kernel(const WT* __restrict__ w,
                                                            const float* __restrict__ out,
                                                            const float* __restrict__ dout,
                                                            float* __restrict__ din, int B,
@@ -81,7 +87,7 @@ __global__ __launch_bounds__(kT) void conv_bwd_data_kernel(const float* __restri
     float acc = 0.0f;
     for (int co = 0; co < Co; co++) {
       const long base = ((long)b * Co + co) * HW;
-      const float* __restrict__ wk = w + ((long)co * Ci + ci) * K * K;
+      const WT* __restrict__ wk = w + ((long)co * Ci + ci) * K * K;
 #pragma unroll
       for (int ky = 0; ky < K; ky++) {
         const int yy = y - ky + P;
@@ -90,7 +96,8 @@ __global__ __launch_bounds__(kT) void conv_bwd_data_kernel(const float* __restri
         for (int kx = 0; kx < K; kx++) {
           const int xx = x - kx + P;
           if (xx < 0 || xx >= W) continue;
-          acc = fmaf(wk[ky * K + kx], relu_grad(out, dout, base + (long)yy * W + xx, relu), acc);
+          acc = fmaf(wload(wk, ky * K + kx), relu_grad(out, dout, base + (long)yy * W + xx, relu),
+                     acc);
         }
       }
     }
@@ -295,6 +302,15 @@ __global__ __launch_bounds__(kT) void adam_kernel(float* __restrict__ p,
   }
 }
 
+// Round to BF16, nearest even (finite values).
+__global__ __launch_bounds__(kT) void to_bf16_kernel(const float* __restrict__ src,
+                                                     unsigned short* __restrict__ dst, long n) {
+  for (long o = blockIdx.x * (long)kT + threadIdx.x; o < n; o += (long)gridDim.x * kT) {
+    const unsigned int u = __float_as_uint(src[o]);
+    dst[o] = (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  }
+}
+
 gvm_status launched(const char* what) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -313,26 +329,28 @@ bool dims_ok(long a, long b, long c, long d, long e) {
 
 using namespace gvm;
 
-extern "C" gvm_status gvm_nn_conv_fwd(gvm_ctx* ctx, const float* in, const float* w,
-                                      const float* bias, float* out, int B, int Ci, int Co, int H,
-                                      int W, int K, int relu) {
+namespace {
+
+template <class WT>
+gvm_status conv_fwd_launch(gvm_ctx* ctx, const float* in, const WT* w, const float* bias,
+                           float* out, int B, int Ci, int Co, int H, int W, int K, int relu) {
   GVM_REQUIRE(ctx && in && w && bias && out, "gvm_nn_conv_fwd: null argument");
   GVM_REQUIRE(K == 1 || K == 3, "gvm_nn_conv_fwd: K must be 1 or 3");
   GVM_REQUIRE(dims_ok(B, Ci, Co, H, W), "gvm_nn_conv_fwd: bad sizes");
   const long n = (long)B * Co * H * W;
   if (K == 3)
-    hipLaunchKernelGGL(conv_fwd_kernel<3>, dim3(nblocks(n)), dim3(kT), 0, ctx->stream, in, w, bias,
-                       out, B, Ci, Co, H, W, relu);
+    hipLaunchKernelGGL((conv_fwd_kernel<3, WT>), dim3(nblocks(n)), dim3(kT), 0, ctx->stream, in, w,
+                       bias, out, B, Ci, Co, H, W, relu);
   else
-    hipLaunchKernelGGL(conv_fwd_kernel<1>, dim3(nblocks(n)), dim3(kT), 0, ctx->stream, in, w, bias,
-                       out, B, Ci, Co, H, W, relu);
+    hipLaunchKernelGGL((conv_fwd_kernel<1, WT>), dim3(nblocks(n)), dim3(kT), 0, ctx->stream, in, w,
+                       bias, out, B, Ci, Co, H, W, relu);
   return launched("gvm_nn_conv_fwd");
 }
 
-extern "C" gvm_status gvm_nn_conv_bwd(gvm_ctx* ctx, const float* in, const float* w,
-                                      const float* out, const float* dout, float* din, float* dw,
-                                      float* dbias, int B, int Ci, int Co, int H, int W, int K,
-                                      int relu) {
+template <class WT>
+gvm_status conv_bwd_launch(gvm_ctx* ctx, const float* in, const WT* w, const float* out,
+                           const float* dout, float* din, float* dw, float* dbias, int B, int Ci,
+                           int Co, int H, int W, int K, int relu) {
   GVM_REQUIRE(ctx && in && w && dout && dw && dbias, "gvm_nn_conv_bwd: null argument");
   GVM_REQUIRE(!relu || out, "gvm_nn_conv_bwd: the ReLU mask needs the forward output");
   GVM_REQUIRE(K == 1 || K == 3, "gvm_nn_conv_bwd: K must be 1 or 3");
@@ -341,11 +359,11 @@ extern "C" gvm_status gvm_nn_conv_bwd(gvm_ctx* ctx, const float* in, const float
   const long nin = (long)B * Ci * H * W;
   if (din) {
     if (K == 3)
-      hipLaunchKernelGGL(conv_bwd_data_kernel<3>, dim3(nblocks(nin)), dim3(kT), 0, ctx->stream, w,
-                         out, dout, din, B, Ci, Co, H, W, relu);
+      hipLaunchKernelGGL((conv_bwd_data_kernel<3, WT>), dim3(nblocks(nin)), dim3(kT), 0,
+                         ctx->stream, w, out, dout, din, B, Ci, Co, H, W, relu);
     else
-      hipLaunchKernelGGL(conv_bwd_data_kernel<1>, dim3(nblocks(nin)), dim3(kT), 0, ctx->stream, w,
-                         out, dout, din, B, Ci, Co, H, W, relu);
+      hipLaunchKernelGGL((conv_bwd_data_kernel<1, WT>), dim3(nblocks(nin)), dim3(kT), 0,
+                         ctx->stream, w, out, dout, din, B, Ci, Co, H, W, relu);
   }
   if (K == 3)
     hipLaunchKernelGGL(conv_bwd_weight_kernel<3>, dim3(Co * Ci), dim3(kT), 0, ctx->stream, in, out,
@@ -356,6 +374,40 @@ extern "C" gvm_status gvm_nn_conv_bwd(gvm_ctx* ctx, const float* in, const float
   hipLaunchKernelGGL(conv_bwd_bias_kernel, dim3(Co), dim3(kT), 0, ctx->stream, out, dout, dbias, B,
                      Co, (long)H * W, relu);
   return launched("gvm_nn_conv_bwd");
+}
+
+}  // namespace
+
+extern "C" gvm_status gvm_nn_conv_fwd(gvm_ctx* ctx, const float* in, const float* w,
+                                      const float* bias, float* out, int B, int Ci, int Co, int H,
+                                      int W, int K, int relu) {
+  return conv_fwd_launch(ctx, in, w, bias, out, B, Ci, Co, H, W, K, relu);
+}
+
+extern "C" gvm_status gvm_nn_conv_fwd_bf16(gvm_ctx* ctx, const float* in,
+                                           const unsigned short* w, const float* bias, float* out,
+                                           int B, int Ci, int Co, int H, int W, int K, int relu) {
+  return conv_fwd_launch(ctx, in, w, bias, out, B, Ci, Co, H, W, K, relu);
+}
+
+extern "C" gvm_status gvm_nn_conv_bwd(gvm_ctx* ctx, const float* in, const float* w,
+                                      const float* out, const float* dout, float* din, float* dw,
+                                      float* dbias, int B, int Ci, int Co, int H, int W, int K,
+                                      int relu) {
+  return conv_bwd_launch(ctx, in, w, out, dout, din, dw, dbias, B, Ci, Co, H, W, K, relu);
+}
+
+extern "C" gvm_status gvm_nn_conv_bwd_bf16(gvm_ctx* ctx, const float* in,
+                                           const unsigned short* w, const float* out,
+                                           const float* dout, float* din, float* dw, float* dbias,
+                                           int B, int Ci, int Co, int H, int W, int K, int relu) {
+  return conv_bwd_launch(ctx, in, w, out, dout, din, dw, dbias, B, Ci, Co, H, W, K, relu);
+}
+
+extern "C" gvm_status gvm_nn_to_bf16(gvm_ctx* ctx, const float* src, unsigned short* dst, long n) {
+  GVM_REQUIRE(ctx && src && dst && n > 0, "gvm_nn_to_bf16: bad argument");
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(nblocks(n)), dim3(kT), 0, ctx->stream, src, dst, n);
+  return launched("gvm_nn_to_bf16");
 }
 
 extern "C" gvm_status gvm_nn_pool_fwd(gvm_ctx* ctx, const float* in, float* out, int B, int C,

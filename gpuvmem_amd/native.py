@@ -144,6 +144,9 @@ HIP_API = {
     "gvm_epoch": [_P, C.POINTER(_L)],
     "gvm_nn_conv_fwd": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I],
     "gvm_nn_conv_bwd": [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I],
+    "gvm_nn_conv_fwd_bf16": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I],
+    "gvm_nn_conv_bwd_bf16": [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I],
+    "gvm_nn_to_bf16": [_P, _P, _P, _L],
     "gvm_nn_pool_fwd": [_P, _P, _P, _I, _I, _I, _I],
     "gvm_nn_pool_bwd": [_P, _P, _P, _I, _I, _I, _I],
     "gvm_nn_up_fwd": [_P, _P, _P, _I, _I, _I, _I, _I, _I],

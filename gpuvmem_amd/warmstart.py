@@ -49,7 +49,7 @@ class UNet:
     H x W (H, W divisible by 2^(levels-1))."""
 
     def __init__(self, ctx: Context, levels: int = 4, base: int = 16, batch: int = 8,
-                 H: int = 256, W: int = 256, seed: int = 0):
+                 H: int = 256, W: int = 256, seed: int = 0, bf16: bool = False):
         if H % (1 << (levels - 1)) or W % (1 << (levels - 1)):
             raise ValueError("image size must be divisible by 2^(levels-1)")
         self.ctx, self.lib = ctx, ctx.lib
@@ -69,6 +69,11 @@ class UNet:
             std = np.sqrt(2.0 / (ci * k * k)) if name != "out" else np.sqrt(1.0 / (ci * k * k))
             flat[wo:bo] = (rng.standard_normal(co * ci * k * k) * std).astype(np.float32)
         self.params = DeviceBuffer.from_array(flat)
+        # BF16 weights: the convolutions read a BF16 copy of the FP32 master
+        # parameters (refreshed after every Adam step); biases stay FP32
+        self.bf16 = bf16
+        self.params_bf16 = DeviceBuffer(2 * off) if bf16 else None
+        self._refresh_bf16()
         self.grads = ctx.zeros(off)
         self.m = ctx.zeros(off)
         self.v = ctx.zeros(off)
@@ -105,6 +110,25 @@ class UNet:
         wo, bo, *_ = self.offsets[name]
         return buf.ptr + 4 * wo, buf.ptr + 4 * bo
 
+    def _wp(self, name):
+        """(weight pointer as the convolutions read it, FP32 bias pointer)."""
+        wo, bo, *_ = self.offsets[name]
+        w = self.params_bf16.ptr + 2 * wo if self.bf16 else self.params.ptr + 4 * wo
+        return w, self.params.ptr + 4 * bo
+
+    def _refresh_bf16(self):
+        if self.bf16:
+            native.check(self.lib.gvm_nn_to_bf16(self.ctx.ctx, self.params.ptr,
+                                                 self.params_bf16.ptr, self.nparams), "to_bf16")
+
+    def _conv_fwd(self, *a):
+        fn = self.lib.gvm_nn_conv_fwd_bf16 if self.bf16 else self.lib.gvm_nn_conv_fwd
+        self._check(fn(self.ctx.ctx, *a), "conv")
+
+    def _conv_bwd(self, *a):
+        fn = self.lib.gvm_nn_conv_bwd_bf16 if self.bf16 else self.lib.gvm_nn_conv_bwd
+        self._check(fn(self.ctx.ctx, *a), "conv bwd")
+
     def _check(self, st, what):
         native.check(st, what)
 
@@ -116,12 +140,10 @@ class UNet:
         cin = 1
         for l in range(L):
             h, w = self.hs[l]
-            wa, ba = self._wb(f"e{l}a", self.params)
-            self._check(lib.gvm_nn_conv_fwd(c, self.enc_in[l].ptr, wa, ba, self.ea[l].ptr, B, cin,
-                                            ch[l], h, w, 3, 1), "conv")
-            wb, bb = self._wb(f"e{l}b", self.params)
-            self._check(lib.gvm_nn_conv_fwd(c, self.ea[l].ptr, wb, bb, self.eb[l].ptr, B, ch[l],
-                                            ch[l], h, w, 3, 1), "conv")
+            wa, ba = self._wp(f"e{l}a")
+            self._conv_fwd(self.enc_in[l].ptr, wa, ba, self.ea[l].ptr, B, cin, ch[l], h, w, 3, 1)
+            wb, bb = self._wp(f"e{l}b")
+            self._conv_fwd(self.ea[l].ptr, wb, bb, self.eb[l].ptr, B, ch[l], ch[l], h, w, 3, 1)
             if l < L - 1:
                 self._check(lib.gvm_nn_pool_fwd(c, self.eb[l].ptr, self.enc_in[l + 1].ptr, B,
                                                 ch[l], h, w), "pool")
@@ -135,41 +157,36 @@ class UNet:
                         "up")
             self._check(lib.gvm_nn_copy_channels(c, self.eb[l].ptr, self.cat[l].ptr, B, ch[l],
                                                  ch[l], 0, ccat, ch[l + 1], h * w), "copy")
-            wa, ba = self._wb(f"d{l}a", self.params)
-            self._check(lib.gvm_nn_conv_fwd(c, self.cat[l].ptr, wa, ba, self.da[l].ptr, B, ccat,
-                                            ch[l], h, w, 3, 1), "conv")
-            wb, bb = self._wb(f"d{l}b", self.params)
-            self._check(lib.gvm_nn_conv_fwd(c, self.da[l].ptr, wb, bb, self.db[l].ptr, B, ch[l],
-                                            ch[l], h, w, 3, 1), "conv")
+            wa, ba = self._wp(f"d{l}a")
+            self._conv_fwd(self.cat[l].ptr, wa, ba, self.da[l].ptr, B, ccat, ch[l], h, w, 3, 1)
+            wb, bb = self._wp(f"d{l}b")
+            self._conv_fwd(self.da[l].ptr, wb, bb, self.db[l].ptr, B, ch[l], ch[l], h, w, 3, 1)
             d = self.db[l]
         self._dout = d
-        wo, bo = self._wb("out", self.params)
-        self._check(lib.gvm_nn_conv_fwd(c, d.ptr, wo, bo, self.y.ptr, B, ch[0], 1, self.H, self.W,
-                                        1, 0), "conv")
+        wo, bo = self._wp("out")
+        self._conv_fwd(d.ptr, wo, bo, self.y.ptr, B, ch[0], 1, self.H, self.W, 1, 0)
 
     # ---- backward (after forward; self.g_y holds dL/dy) ----
     def backward(self) -> None:
         lib, c, B = self.lib, self.ctx.ctx, self.B
         L, ch = self.levels, self.ch
         H, W = self.H, self.W
-        wo, _ = self._wb("out", self.params)
+        wo, _ = self._wp("out")
         gwo, gbo = self._wb("out", self.grads)
-        self._check(lib.gvm_nn_conv_bwd(c, self._dout.ptr, wo, None, self.g_y.ptr, self.g_d[0].ptr,
-                                        gwo, gbo, B, ch[0], 1, H, W, 1, 0), "conv bwd")
+        self._conv_bwd(self._dout.ptr, wo, None, self.g_y.ptr, self.g_d[0].ptr, gwo, gbo, B, ch[0],
+                       1, H, W, 1, 0)
         for l in range(0, L - 1):
             h, w = self.hs[l]
             hs, ws = self.hs[l + 1]
             ccat = ch[l + 1] + ch[l]
-            wb, _ = self._wb(f"d{l}b", self.params)
+            wb, _ = self._wp(f"d{l}b")
             gwb, gbb = self._wb(f"d{l}b", self.grads)
-            self._check(lib.gvm_nn_conv_bwd(c, self.da[l].ptr, wb, self.db[l].ptr, self.g_d[l].ptr,
-                                            self.g_a[l].ptr, gwb, gbb, B, ch[l], ch[l], h, w, 3, 1),
-                        "conv bwd")
-            wa, _ = self._wb(f"d{l}a", self.params)
+            self._conv_bwd(self.da[l].ptr, wb, self.db[l].ptr, self.g_d[l].ptr, self.g_a[l].ptr,
+                           gwb, gbb, B, ch[l], ch[l], h, w, 3, 1)
+            wa, _ = self._wp(f"d{l}a")
             gwa, gba = self._wb(f"d{l}a", self.grads)
-            self._check(lib.gvm_nn_conv_bwd(c, self.cat[l].ptr, wa, self.da[l].ptr, self.g_a[l].ptr,
-                                            self.g_cat[l].ptr, gwa, gba, B, ccat, ch[l], h, w, 3, 1),
-                        "conv bwd")
+            self._conv_bwd(self.cat[l].ptr, wa, self.da[l].ptr, self.g_a[l].ptr, self.g_cat[l].ptr,
+                           gwa, gba, B, ccat, ch[l], h, w, 3, 1)
             # skip part -> gradient at eb[l]; up part -> gradient at the level below's output
             self._check(lib.gvm_nn_copy_channels(c, self.g_cat[l].ptr, self.g_e[l].ptr, B, ch[l],
                                                  ccat, ch[l + 1], ch[l], 0, h * w), "copy")
@@ -183,17 +200,15 @@ class UNet:
                 # eb[l] feeds the skip (g_e[l]) and the pool (g_pool[l])
                 self._check(lib.gvm_nn_add(c, self.g_pool[l].ptr, self.g_e[l].ptr,
                                            B * ch[l] * h * w), "add")
-            wb, _ = self._wb(f"e{l}b", self.params)
+            wb, _ = self._wp(f"e{l}b")
             gwb, gbb = self._wb(f"e{l}b", self.grads)
-            self._check(lib.gvm_nn_conv_bwd(c, self.ea[l].ptr, wb, self.eb[l].ptr, self.g_e[l].ptr,
-                                            self.g_a[l].ptr, gwb, gbb, B, ch[l], ch[l], h, w, 3, 1),
-                        "conv bwd")
-            wa, _ = self._wb(f"e{l}a", self.params)
+            self._conv_bwd(self.ea[l].ptr, wb, self.eb[l].ptr, self.g_e[l].ptr, self.g_a[l].ptr,
+                           gwb, gbb, B, ch[l], ch[l], h, w, 3, 1)
+            wa, _ = self._wp(f"e{l}a")
             gwa, gba = self._wb(f"e{l}a", self.grads)
             gx = self.g_x[l].ptr if l > 0 else None
-            self._check(lib.gvm_nn_conv_bwd(c, self.enc_in[l].ptr, wa, self.ea[l].ptr,
-                                            self.g_a[l].ptr, gx, gwa, gba, B, cin, ch[l], h, w, 3,
-                                            1), "conv bwd")
+            self._conv_bwd(self.enc_in[l].ptr, wa, self.ea[l].ptr, self.g_a[l].ptr, gx, gwa, gba,
+                           B, cin, ch[l], h, w, 3, 1)
             if l > 0:
                 hp, wp = self.hs[l - 1]
                 self._check(lib.gvm_nn_pool_bwd(c, self.g_x[l].ptr, self.g_pool[l - 1].ptr, B,
@@ -213,6 +228,7 @@ class UNet:
         self._check(self.lib.gvm_nn_adam(self.ctx.ctx, self.params.ptr, self.grads.ptr, self.m.ptr,
                                          self.v.ptr, self.nparams, lr, b1, b2, eps, self.step),
                     "adam")
+        self._refresh_bf16()
 
     # ---- host helpers ----
     def set_input(self, xb: np.ndarray) -> None:
@@ -223,11 +239,19 @@ class UNet:
         return self.y.download(np.float32, (self.B, 1, self.H, self.W))
 
     def params_dict(self) -> dict:
+        """The parameters as the convolutions use them (BF16-rounded weights
+        when bf16), in FP64."""
+        self.ctx.sync()
         flat = self.params.download(np.float32).astype(np.float64)
+        if self.bf16:
+            hb = self.params_bf16.download(np.uint16)[: self.nparams]
+            wflat = (hb.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        else:
+            wflat = flat
         out = {}
         for name, co, ci, k in self.shapes:
             wo, bo, *_ = self.offsets[name]
-            out[name] = (flat[wo:bo].reshape(co, ci, k, k), flat[bo:bo + co])
+            out[name] = (wflat[wo:bo].reshape(co, ci, k, k), flat[bo:bo + co])
         return out
 
     def grads_dict(self) -> dict:
@@ -240,8 +264,9 @@ class UNet:
         return out
 
     def close(self) -> None:
-        for b in [self.params, self.grads, self.m, self.v]:
-            b.free()
+        for b in [self.params, self.grads, self.m, self.v, self.params_bf16]:
+            if b is not None:
+                b.free()
 
 
 # ---------------------------------------------------------------------------
@@ -277,13 +302,13 @@ def dirty_image(ctx: Context, p) -> np.ndarray:
 
 
 def solve(p, init: Optional[np.ndarray] = None, iterations: int = 400,
-          optimizer: str = "CG-FRPRMN", graphs: bool = True):
+          optimizer: str = "CG-FRPRMN", graphs: bool = True, adjoint: str = "bf16x3"):
     """The plain MEM optimisation (CG-FRPRMN, the workload's optimiser, or
     CG-LBFGS with memory 10), run to its own tolerance or `iterations`, from
     the default image or `init` (image 0).  Returns (final image 0,
     trajectory of objective values with the starting value first, seconds to
     each entry, result)."""
-    ctx = Context(p)
+    ctx = Context(p, adjoint=adjoint)
     obj = Objective(ctx, fw_config(TERMS, lambdas=LAMBDAS))
     obj.set_graphs(graphs)
     img = p.initial_image()
@@ -324,8 +349,9 @@ class WarmStart:
     """A trained U-Net and its input/output scaling."""
 
     def __init__(self, ctx: Context, levels: int = 4, base: int = 16, batch: int = 8,
-                 npix: int = 256, seed: int = 0):
-        self.net = UNet(ctx, levels=levels, base=base, batch=batch, H=npix, W=npix, seed=seed)
+                 npix: int = 256, seed: int = 0, bf16: bool = True):
+        self.net = UNet(ctx, levels=levels, base=base, batch=batch, H=npix, W=npix, seed=seed,
+                        bf16=bf16)
         self.ctx = ctx
         self.calib = 1.0
         self.history: list = []
@@ -388,10 +414,13 @@ def time_to(traj, secs, target: float):
     return None
 
 
-def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-FRPRMN") -> dict:
-    """L-BFGS from the default image and from the warm start on held-out
-    observations: iterations and seconds (dirty image and inference
-    included for the warm start) to the objective value both reach."""
+def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-FRPRMN",
+             plain: bool = False) -> dict:
+    """The plain solver from the default image and from the warm start on
+    held-out observations: iterations and seconds (dirty image and inference
+    included for the warm start) to the objective value both reach.  With
+    plain=True, also the plain FP32 form (FP32 adjoint, no graphs, default
+    image) to the value all three reach."""
     rows = []
     for s in seeds:
         p = observation(s)
@@ -404,17 +433,29 @@ def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-F
         _, tr_d, sc_d, _ = solve(p, iterations=iterations, optimizer=optimizer)
         img_w, tr_w, sc_w, _ = solve(p, init=init, iterations=iterations, optimizer=optimizer)
         target = max(tr_d[-1], tr_w[-1])
+        extra = {}
+        if plain:
+            _, tr_p, sc_p, _ = solve(p, iterations=iterations, optimizer=optimizer, graphs=False,
+                                     adjoint="f32")
+            t3 = max(target, tr_p[-1])
+            cp, cw = time_to(tr_p, sc_p, t3), time_to(tr_w, sc_w, t3)
+            extra = dict(f_plain=tr_p[-1], iters_plain=cp[0], seconds_plain=cp[1],
+                         seconds_warm_to_common=cw[1] + t_pre)
         a, b = time_to(tr_d, sc_d, target), time_to(tr_w, sc_w, target)
         rows.append(dict(seed=s, f_default=tr_d[-1], f_warm=tr_w[-1], f0_default=tr_d[0],
                          f0_warm=tr_w[0],
                          iters_default=a[0], seconds_default=a[1],
                          iters_warm=b[0], seconds_warm=b[1] + t_pre, warm_overhead_s=t_pre,
-                         iters_run_default=len(tr_d), iters_run_warm=len(tr_w)))
+                         iters_run_default=len(tr_d), iters_run_warm=len(tr_w), **extra))
     it_d = np.array([r["iters_default"] for r in rows], float)
     it_w = np.maximum(np.array([r["iters_warm"] for r in rows], float), 1.0)
     t_d = np.array([r["seconds_default"] for r in rows])
     t_w = np.array([r["seconds_warm"] for r in rows])
-    return dict(rows=rows, median_iter_ratio=float(np.median(it_d / it_w)),
+    out = {}
+    if plain:
+        out["median_time_ratio_plain_fp32"] = float(np.median(
+            [r["seconds_plain"] / r["seconds_warm_to_common"] for r in rows]))
+    return dict(rows=rows, **out, median_iter_ratio=float(np.median(it_d / it_w)),
                 median_time_ratio=float(np.median(t_d / t_w)),
                 mean_iters_default=float(it_d.mean()), mean_iters_warm=float(it_w.mean()),
                 warm_better_or_equal_final=int(sum(r["f_warm"] <= r["f_default"] * (1 + 1e-6)

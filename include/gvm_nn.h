@@ -35,6 +35,17 @@ gvm_status gvm_nn_conv_fwd(gvm_ctx* ctx, const float* in, const float* w, const 
 gvm_status gvm_nn_conv_bwd(gvm_ctx* ctx, const float* in, const float* w, const float* out,
                            const float* dout, float* din, float* dw, float* dbias, int B, int Ci,
                            int Co, int H, int W, int K, int relu);
+/* The same with BF16 weights (uint16 holding the upper half of an FP32,
+ * as gvm_nn_to_bf16 writes them); arithmetic and accumulation stay FP32,
+ * the weight gradients are FP32. */
+gvm_status gvm_nn_conv_fwd_bf16(gvm_ctx* ctx, const float* in, const unsigned short* w,
+                                const float* bias, float* out, int B, int Ci, int Co, int H, int W,
+                                int K, int relu);
+gvm_status gvm_nn_conv_bwd_bf16(gvm_ctx* ctx, const float* in, const unsigned short* w,
+                                const float* out, const float* dout, float* din, float* dw,
+                                float* dbias, int B, int Ci, int Co, int H, int W, int K, int relu);
+/* dst = src rounded to BF16 (nearest even), n values. */
+gvm_status gvm_nn_to_bf16(gvm_ctx* ctx, const float* src, unsigned short* dst, long n);
 /* 2 x 2 average pooling [B][C][H][W] -> [B][C][H/2][W/2] and its backward. */
 gvm_status gvm_nn_pool_fwd(gvm_ctx* ctx, const float* in, float* out, int B, int C, int H, int W);
 gvm_status gvm_nn_pool_bwd(gvm_ctx* ctx, const float* dout, float* din, int B, int C, int H,

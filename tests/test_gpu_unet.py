@@ -60,6 +60,47 @@ def test_conv_kernels_match_oracle(ctx, K, relu):
     assert np.allclose(_get(dbg, b.shape), rb, rtol=1e-5, atol=1e-5)
 
 
+def _bf16_round(w):
+    u = np.ascontiguousarray(w, np.float32).view(np.uint32).astype(np.uint64)
+    h = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return h, (h.astype(np.uint32) << 16).view(np.float32)
+
+
+def test_bf16_weight_conv_matches_oracle(ctx):
+    print("case: conv bf16 weights", flush=True)
+    rng = np.random.default_rng(9)
+    B, Ci, Co, H, W, K = 2, 6, 5, 10, 8, 3
+    x = rng.standard_normal((B, Ci, H, W)).astype(np.float32)
+    w = (rng.standard_normal((Co, Ci, K, K)) * 0.3).astype(np.float32)
+    b = rng.standard_normal(Co).astype(np.float32)
+    wh, wr = _bf16_round(w)
+    # the device rounding equals the host's
+    dwf = _dev(w)
+    dwh = DeviceBuffer(2 * w.size)
+    native.check(ctx.lib.gvm_nn_to_bf16(ctx.ctx, dwf.ptr, dwh.ptr, w.size), "to_bf16")
+    ctx.sync()
+    assert np.array_equal(dwh.download(np.uint16), wh.ravel())
+    dx, db = _dev(x), _dev(b)
+    out = ctx.zeros(B * Co * H * W)
+    native.check(ctx.lib.gvm_nn_conv_fwd_bf16(ctx.ctx, dx.ptr, dwh.ptr, db.ptr, out.ptr, B, Ci, Co,
+                                              H, W, K, 1), "fwd")
+    ref = nn.conv_fwd(x.astype(np.float64), wr.astype(np.float64), b.astype(np.float64), True)
+    ctx.sync()
+    o = _get(out, ref.shape)
+    assert np.allclose(o, ref, rtol=1e-5, atol=1e-5)
+    g = rng.standard_normal(ref.shape).astype(np.float32)
+    din, dwg, dbg = ctx.zeros(x.size), ctx.zeros(w.size), ctx.zeros(b.size)
+    native.check(ctx.lib.gvm_nn_conv_bwd_bf16(ctx.ctx, dx.ptr, dwh.ptr, out.ptr, _dev(g).ptr,
+                                              din.ptr, dwg.ptr, dbg.ptr, B, Ci, Co, H, W, K, 1),
+                 "bwd")
+    rx, rw, rb = nn.conv_bwd(x.astype(np.float64), wr.astype(np.float64), o, g.astype(np.float64),
+                             True)
+    ctx.sync()
+    assert np.allclose(_get(din, x.shape), rx, rtol=1e-5, atol=1e-5)
+    assert np.allclose(_get(dwg, w.shape), rw, rtol=1e-5, atol=1e-4)
+    assert np.allclose(_get(dbg, b.shape), rb, rtol=1e-5, atol=1e-5)
+
+
 def test_pool_up_copy_mse_adam_match_oracle(ctx):
     print("case: pool/up/copy/mse/adam", flush=True)
     rng = np.random.default_rng(2)
@@ -118,10 +159,11 @@ def test_pool_up_copy_mse_adam_match_oracle(ctx):
     assert np.allclose(_get(dp, (50,)), pr, rtol=1e-5, atol=1e-6)
 
 
-def test_unet_forward_backward_match_oracle(ctx):
+@pytest.mark.parametrize("bf16", [False, True])
+def test_unet_forward_backward_match_oracle(ctx, bf16):
     levels, base, B, H = 3, 4, 2, 16
-    print(f"case: unet levels={levels} base={base} {B}x{H}^2", flush=True)
-    net = UNet(ctx, levels=levels, base=base, batch=B, H=H, W=H, seed=3)
+    print(f"case: unet levels={levels} base={base} {B}x{H}^2 bf16={bf16}", flush=True)
+    net = UNet(ctx, levels=levels, base=base, batch=B, H=H, W=H, seed=3, bf16=bf16)
     rng = np.random.default_rng(4)
     x = rng.standard_normal((B, 1, H, H)).astype(np.float32)
     t = rng.standard_normal((B, 1, H, H)).astype(np.float32)
