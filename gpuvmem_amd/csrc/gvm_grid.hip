@@ -170,73 +170,113 @@ __global__ __launch_bounds__(kThreads) void grid_keys_kernel(const double* __res
   }
 }
 
-// One thread per cell (the first entry of each run of equal keys): the float
-// sums of do_gridding in entry order, then the normalisation
-// (src/functions.cu:1518-1559).  Writes, at the run's first entry, whether
-// the cell is kept (weight > 0) and its values.
-__global__ __launch_bounds__(kThreads) void grid_cells_kernel(
-    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ ids, long n, long nvis,
-    GridGeom g, const float* __restrict__ taps, const float* __restrict__ weight,
-    const float2* __restrict__ vo, int* __restrict__ keep, float2* __restrict__ cell_vo,
-    float* __restrict__ cell_w) {
-#pragma clang fp contract(off)
+// Run starts: flag[i] = 1 where a run of equal keys begins (the run of
+// off-grid entries included, so every run ends where the next begins).
+__global__ __launch_bounds__(kThreads) void run_heads_kernel(const uint32_t* __restrict__ keys,
+                                                             long n, int* __restrict__ flag) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    flag[i] = (i == 0 || keys[i - 1] != keys[i]) ? 1 : 0;
+}
+
+// starts[rank[i]] = i at run starts, and starts[number of runs] = n.
+__global__ __launch_bounds__(kThreads) void run_starts_kernel(const int* __restrict__ flag,
+                                                              const int* __restrict__ rank,
+                                                              long n, int* __restrict__ starts) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
-    const uint32_t key = keys[i];
-    if (key == g.off || (i > 0 && keys[i - 1] == key)) {
-      keep[i] = 0;
-      continue;
-    }
-    float ws = 0.f, aux = 0.f, re = 0.f, im = 0.f;
-    for (long p = i; p < n && keys[p] == key; p++) {
-      const uint32_t e = ids[p];
-      const long z = e / g.taps;
-      const int t = (int)(e % g.taps);
+    if (flag[i]) starts[rank[i]] = (int)i;
+    if (i == n - 1) starts[rank[i] + flag[i]] = (int)n;
+  }
+}
+
+// One wave per cell (run r of the sorted entries): the float sums of
+// do_gridding in entry order, then the normalisation
+// (src/functions.cu:1424-1559).  The 64 lanes load a chunk of the run's
+// entries and form the products w k, w k^2, w Vo.x k, w Vo.y k in parallel;
+// lane 0 then adds them in entry order, so the rounding is the reference's
+// single-thread sum.  Writes, at index r, whether the cell is kept
+// (weight > 0) and its values.
+__global__ __launch_bounds__(64) void grid_cells_kernel(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ ids,
+    const int* __restrict__ starts, long nvis, GridGeom g, const float* __restrict__ taps,
+    const float* __restrict__ weight, const float2* __restrict__ vo, int* __restrict__ keep,
+    float2* __restrict__ cell_vo, float* __restrict__ cell_w) {
+#pragma clang fp contract(off)
+  __shared__ float prod[4][64];
+  const int r = blockIdx.x;
+  const int s = starts[r], e = starts[r + 1];
+  const int lane = threadIdx.x;
+  if (keys[s] == g.off) {
+    if (lane == 0) keep[r] = 0;
+    return;
+  }
+  float ws = 0.f, aux = 0.f, re = 0.f, im = 0.f;
+  for (int c = s; c < e; c += 64) {
+    const int p = c + lane;
+    if (p < e) {
+      const uint32_t id = ids[p];
+      const long z = id / g.taps;
+      const int t = (int)(id % g.taps);
       const long vi = z < nvis ? z : z - nvis;
       const float w = weight[vi];
       float2 v = vo[vi];
       if (z >= nvis) v.y = -v.y;
       const float kv = taps[g.kn * (t / g.nx) + (t % g.nx)];
       const float kv2 = kv * kv;
-      ws = ws + w * kv;
-      aux = aux + w * kv2;
-      re = re + w * v.x * kv;
-      im = im + w * v.y * kv;
+      prod[0][lane] = w * kv;
+      prod[1][lane] = w * kv2;
+      prod[2][lane] = w * v.x * kv;
+      prod[3][lane] = w * v.y * kv;
     }
+    __syncthreads();
+    if (lane == 0) {
+      const int cnt = min(64, e - c);
+      for (int j = 0; j < cnt; j++) {
+        ws = ws + prod[0][j];
+        aux = aux + prod[1][j];
+        re = re + prod[2][j];
+        im = im + prod[3][j];
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
     float weff = 0.f;
     float2 out = make_float2(0.f, 0.f);
     if (aux != 0.0f && ws != 0.0f) {
       weff = ws * ws / aux;
       out = make_float2(re / ws, im / ws);
     }
-    keep[i] = weff > 0.0f ? 1 : 0;
-    cell_vo[i] = out;
-    cell_w[i] = weff;
+    keep[r] = weff > 0.0f ? 1 : 0;
+    cell_vo[r] = out;
+    cell_w[r] = weff;
   }
 }
 
 // Kept cells to rows at the cell centre, in row-major cell order
-// (src/functions.cu:1527-1539, 1587-1607): uvw in metres, w = 0.
+// (src/functions.cu:1527-1539, 1587-1607): uvw in metres, w = 0.  One
+// thread per run.
 __global__ __launch_bounds__(kThreads) void grid_emit_kernel(
-    const uint32_t* __restrict__ keys, const int* __restrict__ keep,
-    const int* __restrict__ pos, long n, GridGeom g, const float2* __restrict__ cell_vo,
-    const float* __restrict__ cell_w, long capacity, double* __restrict__ uvw_out,
-    float2* __restrict__ vo_out, float* __restrict__ w_out) {
+    const uint32_t* __restrict__ keys, const int* __restrict__ starts,
+    const int* __restrict__ keep, const int* __restrict__ pos, long nruns, GridGeom g,
+    const float2* __restrict__ cell_vo, const float* __restrict__ cell_w, long capacity,
+    double* __restrict__ uvw_out, float2* __restrict__ vo_out, float* __restrict__ w_out) {
 #pragma clang fp contract(off)
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    if (!keep[i]) continue;
-    const long o = pos[i];
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < nruns;
+       r += (long)gridDim.x * blockDim.x) {
+    if (!keep[r]) continue;
+    const long o = pos[r];
     if (o >= capacity) continue;
-    const uint32_t key = keys[i];
+    const uint32_t key = keys[starts[r]];
     const int gj = (int)(key % (uint32_t)g.N), gk = (int)(key / (uint32_t)g.N);
     const double ul = ((double)gj - g.cj) * g.deltau;
     const double vl = ((double)gk - g.ck) * g.deltav;
     uvw_out[3 * o] = ul * g.lambda;
     uvw_out[3 * o + 1] = vl * g.lambda;
     uvw_out[3 * o + 2] = 0.0;
-    vo_out[o] = cell_vo[i];
-    w_out[o] = cell_w[i];
+    vo_out[o] = cell_vo[r];
+    w_out[o] = cell_w[r];
   }
 }
 
@@ -458,45 +498,65 @@ extern "C" gvm_status gvm_grid_visibilities_dev(gvm_ctx* ctx, const gvm_ckernel*
   GVM_HIP_TRY(hipSetDevice(ctx->device));
 
   gvm_status st;
-  DevMem taps_d, keep, pos, cvo, cw, cnt;
+  DevMem taps_d, flag, rank, starts, keep, pos, cvo, cw;
   CellSort cs;
   if ((st = dev_alloc(taps_d, sizeof(float) * g.km * g.kn, what)) ||
-      (st = cs.reserve(n, what)) || (st = dev_alloc(keep, sizeof(int) * n, what)) ||
-      (st = dev_alloc(pos, sizeof(int) * n, what)) ||
-      (st = dev_alloc(cvo, sizeof(float2) * n, what)) ||
-      (st = dev_alloc(cw, sizeof(float) * n, what)))
+      (st = cs.reserve(n, what)) || (st = dev_alloc(flag, sizeof(int) * n, what)) ||
+      (st = dev_alloc(rank, sizeof(int) * n, what)) ||
+      (st = dev_alloc(starts, sizeof(int) * (n + 1), what)))
     return st;
   GVM_HIP_TRY(hipMemcpy(taps_d.p, kernel->taps, sizeof(float) * g.km * g.kn,
                         hipMemcpyHostToDevice));
   hipLaunchKernelGGL(grid_keys_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, nullptr, uvw_m_dev,
                      nvis, g, cs.in_keys(), cs.in_ids());
   if ((st = launch_ok(what)) || (st = cs.sort(n, key_bits(g.off), what))) return st;
-  hipLaunchKernelGGL(grid_cells_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, nullptr,
-                     cs.keys(), cs.ids(), n, nvis, g, taps_d.as<float>(), weight_dev,
+  // the runs of equal keys (cells, then the off-grid run): start of each
+  hipLaunchKernelGGL(run_heads_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, nullptr, cs.keys(),
+                     n, flag.as<int>());
+  if ((st = launch_ok(what))) return st;
+  size_t sbytes = 0;
+  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, sbytes, flag.as<int>(), rank.as<int>(),
+                                               (int)n));
+  DevMem stmp;
+  if ((st = dev_alloc(stmp, sbytes, what))) return st;
+  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(stmp.p, sbytes, flag.as<int>(), rank.as<int>(),
+                                               (int)n));
+  int last[2] = {0, 0};
+  GVM_HIP_TRY(hipMemcpy(&last[0], rank.as<int>() + n - 1, sizeof(int), hipMemcpyDeviceToHost));
+  GVM_HIP_TRY(hipMemcpy(&last[1], flag.as<int>() + n - 1, sizeof(int), hipMemcpyDeviceToHost));
+  const long nruns = (long)last[0] + last[1];
+  hipLaunchKernelGGL(run_starts_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, nullptr,
+                     flag.as<int>(), rank.as<int>(), n, starts.as<int>());
+  if ((st = launch_ok(what)) || (st = dev_alloc(keep, sizeof(int) * nruns, what)) ||
+      (st = dev_alloc(pos, sizeof(int) * nruns, what)) ||
+      (st = dev_alloc(cvo, sizeof(float2) * nruns, what)) ||
+      (st = dev_alloc(cw, sizeof(float) * nruns, what)))
+    return st;
+  hipLaunchKernelGGL(grid_cells_kernel, dim3((unsigned)nruns), dim3(64), 0, nullptr, cs.keys(),
+                     cs.ids(), starts.as<int>(), nvis, g, taps_d.as<float>(), weight_dev,
                      reinterpret_cast<const float2*>(vo_dev), keep.as<int>(), cvo.as<float2>(),
                      cw.as<float>());
   if ((st = launch_ok(what))) return st;
   // exclusive prefix sum of the keep flags: output row of each kept cell
-  size_t sbytes = 0;
-  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, sbytes, keep.as<int>(), pos.as<int>(),
-                                               (int)n));
-  DevMem stmp;
-  if ((st = dev_alloc(stmp, sbytes, what))) return st;
-  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(stmp.p, sbytes, keep.as<int>(), pos.as<int>(),
-                                               (int)n));
-  int last[2] = {0, 0};
-  GVM_HIP_TRY(hipMemcpy(&last[0], pos.as<int>() + n - 1, sizeof(int), hipMemcpyDeviceToHost));
-  GVM_HIP_TRY(hipMemcpy(&last[1], keep.as<int>() + n - 1, sizeof(int), hipMemcpyDeviceToHost));
+  size_t kbytes = 0;
+  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, kbytes, keep.as<int>(), pos.as<int>(),
+                                               (int)nruns));
+  if (kbytes > sbytes && (st = dev_alloc(stmp, kbytes, what))) return st;
+  kbytes = std::max(kbytes, sbytes);
+  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(stmp.p, kbytes, keep.as<int>(), pos.as<int>(),
+                                               (int)nruns));
+  GVM_HIP_TRY(hipMemcpy(&last[0], pos.as<int>() + nruns - 1, sizeof(int), hipMemcpyDeviceToHost));
+  GVM_HIP_TRY(hipMemcpy(&last[1], keep.as<int>() + nruns - 1, sizeof(int), hipMemcpyDeviceToHost));
   const long count = (long)last[0] + last[1];
   if (count > capacity) {
     set_error("gvm_grid_visibilities_dev: " + std::to_string(count) +
               " gridded rows exceed the output capacity " + std::to_string(capacity));
     return GVM_ERR_ARG;
   }
-  hipLaunchKernelGGL(grid_emit_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, nullptr,
-                     cs.keys(), keep.as<int>(), pos.as<int>(), n, g, cvo.as<float2>(),
-                     cw.as<float>(), capacity, uvw_out_dev, reinterpret_cast<float2*>(vo_out_dev),
-                     weight_out_dev);
+  hipLaunchKernelGGL(grid_emit_kernel, dim3(blocks_for(nruns)), dim3(kThreads), 0, nullptr,
+                     cs.keys(), starts.as<int>(), keep.as<int>(), pos.as<int>(), nruns, g,
+                     cvo.as<float2>(), cw.as<float>(), capacity, uvw_out_dev,
+                     reinterpret_cast<float2*>(vo_out_dev), weight_out_dev);
   if ((st = launch_ok(what))) return st;
   GVM_HIP_TRY(hipDeviceSynchronize());
   *nout = count;
