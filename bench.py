@@ -274,17 +274,22 @@ def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
     earlier when its line search makes no progress).  The target is the
     objective value every form reached (the largest of the final values),
     and each form is timed to its first iteration at or below it.  Each
-    context does one untimed evaluation first (workspace allocation)."""
+    context does one untimed evaluation first (workspace allocation).  The
+    plain FP32 form runs as it is; the split forms also replay evaluation
+    graphs and use the autotuned adjoint range length."""
     from gpuvmem_amd.device import Context, Objective, fw_config
 
     terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
     out, runs, images = {}, {}, {}
     for mode in forms:
+        extras = mode != "f32"
         ctx = Context(p, adjoint=mode)
         obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
+        obj.set_graphs(extras)
         warm = ctx.image(p.initial_image())
         xi = ctx.zeros(2 * p.M * p.N)
         obj.value(warm)
+        tune = ctx.autotune() if extras else None
         obj.gradient(warm, xi, 0)
         ctx.sync()
         d = ctx.image(p.initial_image())
@@ -301,6 +306,10 @@ def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
             "final_chi2": chi2,
             "function_calls": res.function_calls,
             "gradient_calls": res.gradient_calls,
+            "graphs": extras,
+            "graph_replays": obj.graph_launches(),
+            "split_vis": ctx.split_vis() if mode != "direct" else None,
+            "autotune": tune,
         }
         obj.close()
         ctx.close()
