@@ -395,6 +395,9 @@ def main() -> int:
                     help="replay captured HIP graphs of the value and gradient evaluations")
     ap.add_argument("--warm-start", action="store_true",
                     help="train the U-Net warm start and measure its gain (256^2 observations)")
+    ap.add_argument("--ws-pairs", type=int, default=96, help="warm start: training pairs")
+    ap.add_argument("--ws-steps", type=int, default=400, help="warm start: Adam steps")
+    ap.add_argument("--ws-test", type=int, default=12, help="warm start: held-out observations")
     ap.add_argument("--gridding", action="store_true",
                     help="time the device weighting and gridder and the gridded objective step")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -409,7 +412,8 @@ def main() -> int:
     from gpuvmem_amd.problem import make_mfs_problem, make_problem
 
     if args.warm_start:
-        r = bench_warm_start(n_train=96, n_test=12, steps=400, iterations=500, levels=4, base=16)
+        r = bench_warm_start(n_train=args.ws_pairs, n_test=args.ws_test, steps=args.ws_steps,
+                             iterations=500, levels=4, base=16)
         print(json.dumps(r), flush=True)
         return 0
     t0 = time.perf_counter()
