@@ -653,6 +653,15 @@ extern "C" gvm_status gvm_set_split_vis(gvm_ctx* ctx, long split_vis) {
   return GVM_OK;
 }
 
+extern "C" gvm_status gvm_set_fft_mode(gvm_ctx* ctx, int mode) {
+  GVM_REQUIRE(ctx, "gvm_set_fft_mode: null context");
+  GVM_REQUIRE(mode >= 0 && mode <= 2, "gvm_set_fft_mode: mode must be 0, 1 or 2");
+  GVM_REQUIRE(!ctx->capturing, "gvm_set_fft_mode: capturing");
+  if (ctx->fft_mode != mode) ctx->epoch++;
+  ctx->fft_mode = mode;
+  return GVM_OK;
+}
+
 extern "C" gvm_status gvm_get_split_vis(const gvm_ctx* ctx, long* split_vis) {
   GVM_REQUIRE(ctx && split_vis, "gvm_get_split_vis: null argument");
   *split_vis = ctx->split_vis;

@@ -54,7 +54,11 @@ __global__ void zero_u32(unsigned int* p, int n) {
 // ---------------------------------------------------------------------------
 // Stockham FFT in LDS
 // ---------------------------------------------------------------------------
+// Complex product, each operation rounded (no multiply-add contraction), so
+// the transform's rounding does not depend on how the compiler schedules the
+// unrolled stages of a given batching.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+#pragma clang fp contract(off)
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
@@ -63,17 +67,18 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 //   rows: element (t, p) at t * N + p;  cols: element (t, p) at p * B + t.
 // All reads of the stage complete (barrier) before any write, so the stage
 // runs in place in one buffer.
-template <int N, int B, bool COLS, int R, int NS>
+template <int N, int B, bool COLS, int R, int NS, int T>
 __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __restrict__ tw) {
+#pragma clang fp contract(off)
   constexpr int NB = B * N / R;          // butterflies in this stage
-  constexpr int BPT = NB / 256;          // per thread
-  static_assert(NB % 256 == 0, "butterflies must divide evenly over 256 threads");
+  constexpr int BPT = NB / T;            // per thread
+  static_assert(NB % T == 0, "butterflies must divide evenly over the threads");
   constexpr int STRIDE = N / R;
   float2 v[BPT][R];
   int base_t[BPT], out_j[BPT];
 #pragma unroll
   for (int b = 0; b < BPT; b++) {
-    const int q = threadIdx.x + b * 256;
+    const int q = threadIdx.x + b * T;
     const int t = COLS ? (q % B) : (q / STRIDE);
     const int j = COLS ? (q / B) : (q % STRIDE);
     const int k = j % NS;
@@ -124,11 +129,11 @@ __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __rest
   __syncthreads();
 }
 
-template <int N, int B, bool COLS, int NS>
+template <int N, int B, bool COLS, int NS, int T>
 __device__ __forceinline__ void radix4_stages(float2* lds, const float2* __restrict__ tw) {
   if constexpr (NS < N) {
-    stockham_stage<N, B, COLS, 4, NS>(lds, tw);
-    radix4_stages<N, B, COLS, NS * 4>(lds, tw);
+    stockham_stage<N, B, COLS, 4, NS, T>(lds, tw);
+    radix4_stages<N, B, COLS, NS * 4, T>(lds, tw);
   }
 }
 
@@ -143,13 +148,13 @@ constexpr bool log2_is_odd() {
   return (l & 1) != 0;
 }
 
-template <int N, int B, bool COLS>
+template <int N, int B, bool COLS, int T>
 __device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ tw) {
   if constexpr (log2_is_odd<N>()) {
-    stockham_stage<N, B, COLS, 2, 1>(lds, tw);
-    radix4_stages<N, B, COLS, 2>(lds, tw);
+    stockham_stage<N, B, COLS, 2, 1, T>(lds, tw);
+    radix4_stages<N, B, COLS, 2, T>(lds, tw);
   } else {
-    radix4_stages<N, B, COLS, 1>(lds, tw);
+    radix4_stages<N, B, COLS, 1, T>(lds, tw);
   }
 }
 
@@ -157,8 +162,8 @@ __device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ 
 // (calculateInu + apply_beam2I, src/functions.cu:3939-3966, 2424-2444),
 // transform each row, multiply by rot_x[column frequency], write the rows of
 // that slice's grid.
-template <int N, int B>
-__global__ __launch_bounds__(256) void fft_rows_kernel(
+template <int N, int B, int T>
+__global__ __launch_bounds__(T) void fft_rows_kernel(
     const float* __restrict__ I, const SliceDev* __restrict__ tab, float fg_scale,
     float floor_v, const float* __restrict__ gcf, const float2* __restrict__ tw,
     const float2* __restrict__ rot_x, float2* __restrict__ grids) {
@@ -168,7 +173,7 @@ __global__ __launch_bounds__(256) void fft_rows_kernel(
   const float nudiv = tab[blockIdx.y].nudiv;
   float2* __restrict__ grid = grids + (long)blockIdx.y * n;
   const int row0 = blockIdx.x * B;
-  for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+  for (int idx = threadIdx.x; idx < B * N; idx += T) {
     const long k = (long)(row0 + idx / N) * N + (idx % N);
     float inu = I[k] * powf(nudiv, I[n + k]);
     if (inu < floor_v) inu = floor_v;
@@ -177,46 +182,65 @@ __global__ __launch_bounds__(256) void fft_rows_kernel(
     lds[idx] = make_float2(x, 0.0f);
   }
   __syncthreads();
-  fft_lds<N, B, false>(lds, tw);
-  for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+  fft_lds<N, B, false, T>(lds, tw);
+  for (int idx = threadIdx.x; idx < B * N; idx += T) {
     const int j = idx % N;
     grid[(long)(row0 + idx / N) * N + j] = cmul(lds[idx], rot_x[j]);
   }
 }
 
 // Column pass: B adjacent columns per block of slice blockIdx.y, in place.
-template <int N, int B>
-__global__ __launch_bounds__(256) void fft_cols_kernel(float2* __restrict__ grids,
+template <int N, int B, int T>
+__global__ __launch_bounds__(T) void fft_cols_kernel(float2* __restrict__ grids,
                                                        const float2* __restrict__ tw,
                                                        const float2* __restrict__ rot_y) {
   __shared__ float2 lds[B * N];
   float2* __restrict__ grid = grids + (long)blockIdx.y * N * N;
   const int col0 = blockIdx.x * B;
-  for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+  for (int idx = threadIdx.x; idx < B * N; idx += T) {
     const int p = idx / B, t = idx % B;
     lds[idx] = grid[(long)p * N + col0 + t];
   }
   __syncthreads();
-  fft_lds<N, B, true>(lds, tw);
-  for (int idx = threadIdx.x; idx < B * N; idx += 256) {
+  fft_lds<N, B, true, T>(lds, tw);
+  for (int idx = threadIdx.x; idx < B * N; idx += T) {
     const int p = idx / B, t = idx % B;
     grid[(long)p * N + col0 + t] = cmul(lds[idx], rot_y[p]);
   }
 }
 
+template <int N, int BR, int TR, int BC, int TC>
+gvm_status launch_fft_with(gvm_ctx* ctx, const float* I, int nsl, float floor_v) {
+  hipLaunchKernelGGL((fft_rows_kernel<N, BR, TR>), dim3(N / BR, nsl), dim3(TR), 0, ctx->stream, I,
+                     ctx->slice_tab, ctx->g.fg_scale, floor_v, ctx->gcf, ctx->twiddle,
+                     ctx->rot_x, ctx->grid);
+  GVM_CHECK_LAUNCH();
+  hipLaunchKernelGGL((fft_cols_kernel<N, BC, TC>), dim3(N / BC, nsl), dim3(TC), 0, ctx->stream,
+                     ctx->grid, ctx->twiddle, ctx->rot_y);
+  GVM_CHECK_LAUNCH();
+  return GVM_OK;
+}
+
+// Two batchings of the same transforms (identical arithmetic, so identical
+// results): "wide" blocks of 256 threads over 4096 / N rows and 8192 / N
+// columns, for many slices; "narrow" blocks of 1 - 4 rows or 2 - 4 columns
+// with 64 - 256 threads, so a single small image still spreads over the
+// CUs.  Automatic choice (ctx->fft_mode 0): narrow when the wide row pass
+// would launch fewer than 512 blocks.
 template <int N>
 gvm_status launch_fft(gvm_ctx* ctx, const float* I, int nsl, float floor_v) {
   constexpr int BR = (4096 / N) > 0 ? (4096 / N) : 1;
   constexpr int BC0 = 8192 / N;
   constexpr int BC = BC0 < 2 ? 2 : (BC0 > N ? N : BC0);
-  hipLaunchKernelGGL((fft_rows_kernel<N, BR>), dim3(N / BR, nsl), dim3(256), 0, ctx->stream, I,
-                     ctx->slice_tab, ctx->g.fg_scale, floor_v, ctx->gcf, ctx->twiddle,
-                     ctx->rot_x, ctx->grid);
-  GVM_CHECK_LAUNCH();
-  hipLaunchKernelGGL((fft_cols_kernel<N, BC>), dim3(N / BC, nsl), dim3(256), 0, ctx->stream,
-                     ctx->grid, ctx->twiddle, ctx->rot_y);
-  GVM_CHECK_LAUNCH();
-  return GVM_OK;
+  if constexpr (N <= 1024) {
+    constexpr int BRn = 256 / N > 1 ? 256 / N : 1;
+    constexpr int TRn = BRn * N / 4 < 256 ? BRn * N / 4 : 256;
+    constexpr int BCn = 256 / N > 2 ? 256 / N : 2;
+    constexpr int TCn = BCn * N / 4 < 256 ? BCn * N / 4 : 256;
+    const bool narrow = ctx->fft_mode == 2 || (ctx->fft_mode == 0 && (long)nsl * (N / BR) < 512);
+    if (narrow) return launch_fft_with<N, BRn, TRn, BCn, TCn>(ctx, I, nsl, floor_v);
+  }
+  return launch_fft_with<N, BR, 256, BC, 256>(ctx, I, nsl, floor_v);
 }
 
 // The phase-rotated uv grids of all slices (ctx->grid, one N x N grid each).

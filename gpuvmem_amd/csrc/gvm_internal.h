@@ -110,6 +110,7 @@ struct gvm_ctx {
   int split_mode = -1;          // adjoint mode the table was built for (-1: stale)
   int adjoint_mode = GVM_ADJOINT_BF16X3;
   long split_vis = 3072;        // target visibilities per adjoint range (K tile)
+  int fft_mode = 0;             // FFT batching: 0 automatic, 1 wide, 2 narrow
   std::vector<gvm::Slice> slices;
   bool have_residuals = false;
   // graph capture: true between gvm_capture_begin and gvm_capture_end; the

@@ -186,6 +186,10 @@ class Context:
         native.check(self.lib.gvm_values_host(self.ctx, n, out), "gvm_values_host")
         return list(out)
 
+    def set_fft_mode(self, mode: str) -> None:
+        native.check(self.lib.gvm_set_fft_mode(self.ctx, {"auto": 0, "wide": 1, "narrow": 2}[mode]),
+                     "gvm_set_fft_mode")
+
     def set_split_vis(self, n: int) -> None:
         native.check(self.lib.gvm_set_split_vis(self.ctx, n), "gvm_set_split_vis")
 

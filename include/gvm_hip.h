@@ -401,6 +401,11 @@ gvm_status gvm_timing_adjoint_ms(gvm_ctx* ctx, double* avg_ms, long* count);
  * ranges keep the FP32 accumulation short, longer ones write fewer partial
  * tiles.  Default 3072; allowed 256 .. 16384. */
 gvm_status gvm_set_split_vis(gvm_ctx* ctx, long split_vis);
+/* Batching of the image-to-grid transform (no counterpart in the reference,
+ * which calls cuFFT): 0 automatic (default), 1 wide blocks (many rows or
+ * columns per block, for many slices), 2 narrow blocks (one to four, for a
+ * single small image).  Both give identical grids. */
+gvm_status gvm_set_fft_mode(gvm_ctx* ctx, int mode);
 gvm_status gvm_get_split_vis(const gvm_ctx* ctx, long* split_vis);
 /* Autotuner: times the adjoint of the current residuals (a gvm_chi2 must
  * have run) for each candidate range length (NULL: 1536, 2048, 3072, 4096,
