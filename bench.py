@@ -398,6 +398,8 @@ def main() -> int:
     ap.add_argument("--ws-pairs", type=int, default=96, help="warm start: training pairs")
     ap.add_argument("--ws-steps", type=int, default=400, help="warm start: Adam steps")
     ap.add_argument("--ws-test", type=int, default=12, help="warm start: held-out observations")
+    ap.add_argument("--ws-no-graphs", action="store_true",
+                    help="warm start: the solver launches every kernel (no evaluation graphs)")
     ap.add_argument("--gridding", action="store_true",
                     help="time the device weighting and gridder and the gridded objective step")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -412,6 +414,10 @@ def main() -> int:
     from gpuvmem_amd.problem import make_mfs_problem, make_problem
 
     if args.warm_start:
+        if args.ws_no_graphs:
+            from gpuvmem_amd import warmstart
+
+            warmstart.GRAPHS = False
         r = bench_warm_start(n_train=args.ws_pairs, n_test=args.ws_test, steps=args.ws_steps,
                              iterations=500, levels=4, base=16)
         print(json.dumps(r), flush=True)

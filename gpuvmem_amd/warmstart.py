@@ -276,6 +276,8 @@ class UNet:
 # the objective of the 256^2 workload: chi2 + Entropy (lambda 0.01, G = 0.001)
 TERMS = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001)]
 LAMBDAS = (0.01,)
+# whether the solver replays evaluation graphs (the default of solve())
+GRAPHS = True
 
 
 def observation(seed: int, **overrides):
@@ -302,7 +304,7 @@ def dirty_image(ctx: Context, p) -> np.ndarray:
 
 
 def solve(p, init: Optional[np.ndarray] = None, iterations: int = 400,
-          optimizer: str = "CG-FRPRMN", graphs: bool = True, adjoint: str = "bf16x3"):
+          optimizer: str = "CG-FRPRMN", graphs: Optional[bool] = None, adjoint: str = "bf16x3"):
     """The plain MEM optimisation (CG-FRPRMN, the workload's optimiser, or
     CG-LBFGS with memory 10), run to its own tolerance or `iterations`, from
     the default image or `init` (image 0).  Returns (final image 0,
@@ -310,7 +312,7 @@ def solve(p, init: Optional[np.ndarray] = None, iterations: int = 400,
     each entry, result)."""
     ctx = Context(p, adjoint=adjoint)
     obj = Objective(ctx, fw_config(TERMS, lambdas=LAMBDAS))
-    obj.set_graphs(graphs)
+    obj.set_graphs(GRAPHS if graphs is None else graphs)
     img = p.initial_image()
     if init is not None:
         img[0] = np.maximum(init, p.minpix).astype(np.float32)
