@@ -140,6 +140,34 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned int* hi,
   }
 }
 
+// Load through a global-address-space pointer.  Pointers read from device
+// tables are generic, and generic (flat) loads also count against the LDS
+// wait counter, so every LDS wait would wait for them too.
+// (HIP's float2 / float4 are classes whose copy goes through a generic
+// reference, so they are loaded as native vectors.)
+typedef float gf2_t __attribute__((ext_vector_type(2)));
+typedef float gf4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float ldg(const float* p) {
+  return *(const __attribute__((address_space(1))) float*)p;
+}
+__device__ __forceinline__ float2 ldg(const float2* p) {
+  const gf2_t v = *(const __attribute__((address_space(1))) gf2_t*)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 ldg(const float4* p) {
+  const gf4_t v = *(const __attribute__((address_space(1))) gf4_t*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Workgroup barrier that orders LDS accesses only: global loads issued
+// before it (prefetches into registers) stay in flight across it, where
+// __syncthreads would wait for them.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int kGemmThreads = 512;
 constexpr int kPScale = 15;  // max |P| = 2^15 < 65504 (FP16 max)
 constexpr int kHalfKV = kKV / 2;
@@ -207,18 +235,25 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
   const int lr = lane & 31, lh = lane >> 5;
   const int wr = (wave & 1) * 128, wc = (wave >> 1) * 64;
 
-  // ---- visibility records (wave 0; lane l handles slot l % kKV) ----
+  // ---- visibility records (wave 0; lane l handles slot l % kKV): loaded
+  //      into registers one chunk before they are staged; the arithmetic on
+  //      them waits until then, so the loads stay in flight ----
   const int slot = lane & (kKV - 1);
-  auto load_rec = [&](int c, float4& s, float4& d) {
+  float4 rs;
+  float2 rp;
+  float rw;
+  bool rin;
+  auto load_rec = [&](int c) {
     const long k = kbeg + (long)c * kKV + slot;
     const long kc = min(k, nvis - 1);
-    s = phs[kc];
-    const float2 p = cpol[kc];
-    d = make_float4(k < kend ? p.x * scale : 0.0f, p.y, wcoord[kc], 0.f);
+    rs = ldg(phs + kc);
+    rp = ldg(cpol + kc);
+    rw = ldg(wcoord + kc);
+    rin = k < kend;
   };
-  auto store_rec = [&](int buf, const float4& s, const float4& d) {
-    sm.rec_s[buf][slot] = s;
-    sm.rec_d[buf][slot] = d;
+  auto store_rec = [&](int buf) {
+    sm.rec_s[buf][slot] = rs;
+    sm.rec_d[buf][slot] = make_float4(rin ? rp.x * scale : 0.0f, rp.y, rw, 0.f);
   };
 
   // ---- generation: P row gidx and Q column gidx for half of the chunk ----
@@ -327,14 +362,12 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
 
   if (nchunks > 0) {
     // prologue: records of chunks 0 and 1, tiles of chunk 0
-    float4 sn, dn;
     if (wave == 0) {
-      float4 s1, d1;
-      load_rec(0, sn, dn);
-      load_rec(1, s1, d1);
-      store_rec(0, sn, dn);
-      store_rec(1, s1, d1);
-      load_rec(2, sn, dn);
+      load_rec(0);
+      store_rec(0);
+      load_rec(1);
+      store_rec(1);
+      load_rec(2);
     }
     __syncthreads();
     generate(0);
@@ -350,10 +383,10 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
         generate(nxt);
       }
       if (wave == 0) {
-        store_rec(cur, sn, dn);  // records of chunk c+2
-        load_rec(c + 3, sn, dn);
+        store_rec(cur);  // records of chunk c+2
+        load_rec(c + 3);
       }
-      __syncthreads();
+      lds_barrier();
     }
   }
 
