@@ -180,11 +180,12 @@ def bench_gridding(p, steps: int, warmup: int, threads: int, adjoint: str) -> di
 
 
 def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, levels: int,
-                     base: int) -> dict:
-    """The learned warm start at the 256^2 / 36,000-visibility scale: training
-    pairs from the plain solver (CG-FRPRMN to its tolerance), U-Net training on
-    the device kernels, then the plain solver from the default image and from
-    the prediction on held-out observations."""
+                     base: int, which: int = 0, batch: int = 8) -> dict:
+    """The learned warm start on observations of workload `which` (0: 256^2 /
+    36,000 visibilities; 4: 512^2 / 1,029,000): training pairs from the plain
+    solver (CG-FRPRMN to its tolerance), U-Net training on the device kernels,
+    then the plain solver from the default image and from the prediction on
+    held-out observations."""
     import numpy as np
     from gpuvmem_amd import warmstart as W
     from gpuvmem_amd.device import Context
@@ -193,18 +194,24 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
     chunks = []
     for c0 in range(0, n_train, 16):
         chunks.append(W.make_pairs(range(1000 + c0, 1000 + min(n_train, c0 + 16)),
-                                   iterations=iterations))
+                                   iterations=iterations, which=which))
         print(f"# pairs {c0 + len(chunks[-1].dirty)} ({time.perf_counter() - t0:.1f} s)",
               file=sys.stderr, flush=True)
     pairs = W.Pairs(np.concatenate([c.dirty for c in chunks]),
                     np.concatenate([c.model for c in chunks]))
     t_pairs = time.perf_counter() - t0
-    p = W.observation(1000)
+    p = W.observation(1000, which)
     ctx = Context(p, add_data=False)
-    ws = W.WarmStart(ctx, levels=levels, base=base, batch=8, npix=p.M)
-    t_train = ws.train(pairs, steps=steps)
+    ws = W.WarmStart(ctx, levels=levels, base=base, batch=batch, npix=p.M)
+    t_train = ws.train(pairs, steps=steps, log_every=25, progress=lambda k, loss: print(
+        f"# step {k} loss {loss:.3e} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+        flush=True))
     print(f"# trained in {t_train:.1f} s, loss {ws.history[-1][1]:.3e}", file=sys.stderr, flush=True)
-    r = W.evaluate(ws, range(5000, 5000 + n_test), iterations=iterations, plain=True)
+    r = W.evaluate(ws, range(5000, 5000 + n_test), iterations=iterations, plain=True, which=which,
+                   progress=lambda row: print(
+                       f"# held-out {row['seed']}: iterations {row['iters_default']} / "
+                       f"{row['iters_warm']}, seconds {row['seconds_default']:.2f} / "
+                       f"{row['seconds_warm']:.2f}", file=sys.stderr, flush=True))
     rows = r.pop("rows")
     out = {
         "metric": "learned warm start: time to the objective both starts reach, default / warm",
@@ -214,7 +221,7 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
                                f"CG-FRPRMN to its tolerance (at most {iterations} iterations)",
                    "unet": {"levels": levels, "base_channels": base, "parameters": ws.net.nparams,
                             "weights": "bf16 (fp32 master, fp32 accumulation)" if ws.net.bf16
-                            else "fp32", "batch": 8},
+                            else "fp32", "batch": batch},
                    "train_pairs": n_train, "adam_steps": steps, "held_out": n_test},
         "pairs_seconds": t_pairs, "train_seconds": t_train,
         "train_loss": ws.history, **r,
@@ -400,6 +407,14 @@ def main() -> int:
     ap.add_argument("--ws-test", type=int, default=12, help="warm start: held-out observations")
     ap.add_argument("--ws-no-graphs", action="store_true",
                     help="warm start: the solver launches every kernel (no evaluation graphs)")
+    ap.add_argument("--ws-workload", type=int, default=0, choices=(0, 4),
+                    help="warm start: observations of workload 0 (256^2, 36,000 vis) or 4 "
+                         "(512^2, 1,029,000 vis)")
+    ap.add_argument("--ws-levels", type=int, default=None,
+                    help="warm start: U-Net levels (default 4 at 256^2, 6 at 512^2)")
+    ap.add_argument("--ws-base", type=int, default=None,
+                    help="warm start: U-Net base channels (default 16 at 256^2, 32 at 512^2)")
+    ap.add_argument("--ws-batch", type=int, default=8, help="warm start: training batch")
     ap.add_argument("--gridding", action="store_true",
                     help="time the device weighting and gridder and the gridded objective step")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -418,8 +433,12 @@ def main() -> int:
             from gpuvmem_amd import warmstart
 
             warmstart.GRAPHS = False
+        big = args.ws_workload == 4
         r = bench_warm_start(n_train=args.ws_pairs, n_test=args.ws_test, steps=args.ws_steps,
-                             iterations=500, levels=4, base=16)
+                             iterations=500,
+                             levels=args.ws_levels or (6 if big else 4),
+                             base=args.ws_base or (32 if big else 16),
+                             which=args.ws_workload, batch=args.ws_batch)
         print(json.dumps(r), flush=True)
         return 0
     t0 = time.perf_counter()

@@ -313,6 +313,29 @@ extern "C" void gvm_sim_default_config(int which, gvm_sim_config* c) {
     c->seed_sky = 31;
     c->seed_noise = 32;
     c->seed_flag = 33;
+  } else if (which == 4) {
+    // 512^2 warm-start workload: the 256^2 workload's skies scaled up (5
+    // Gaussian blobs of 4-20 px, log-uniform fluxes 0.01-1 Jy, 3 points) seen
+    // by 50 antennas uniform in a 1 km disk, 2 h 20 min at 10 s (1225 x 840 =
+    // 1,029,000 rows), 230 GHz.
+    c->npix = 512;
+    c->n_antennas = 50;
+    c->layout = GVM_SIM_LAYOUT_UNIFORM;
+    c->array_diameter_m = 1000.0;
+    c->ha_start_hours = -7.0 / 6.0;
+    c->ha_end_hours = 7.0 / 6.0;
+    c->dump_seconds = 10.0;
+    c->freq_hz = 230e9;
+    c->n_gaussians = 5;
+    c->n_points = 3;
+    c->sigma_min_pix = 4.0;
+    c->sigma_max_pix = 20.0;
+    c->flux_law = GVM_SIM_FLUX_LOGUNIFORM;
+    c->flux_min = 0.01;
+    c->flux_max = 1.0;
+    c->seed_layout = 40;
+    c->seed_sky = 41;
+    c->seed_noise = 42;
   } else if (which == 1) {
     // 1024^2; 50 antennas Gaussian-radial in a 2 km disk; 4 h at 10 s;
     // 200 Gaussian components with power-law fluxes.

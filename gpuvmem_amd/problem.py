@@ -103,8 +103,8 @@ def sim_config(which: int = 0, **overrides) -> native.SimConfig:
 
 def make_problem(which: int = 0, noise_cut_factor: float = 10.0, **overrides) -> Problem:
     """Generates workload ``which`` (0: 256^2 / 36,000 vis; 1: 1024^2 /
-    1,764,000 vis; 2: 2048^2 / 4,354,560 vis) with optional field overrides
-    and runs the set-up."""
+    1,764,000 vis; 2: 2048^2 / 4,354,560 vis; 4: 512^2 / 1,029,000 vis, the
+    warm start's) with optional field overrides and runs the set-up."""
     fw = native.load_fw()
     cfg = sim_config(which, **overrides)
     nvis = fw.gvm_sim_count_visibilities(C.byref(cfg))

@@ -280,14 +280,15 @@ LAMBDAS = (0.01,)
 GRAPHS = True
 
 
-def observation(seed: int, **overrides):
-    """A 256^2 / 36,000-visibility observation of the workload's array with
-    the sky and the noise drawn from `seed` (the array layout stays fixed)."""
+def observation(seed: int, which: int = 0, **overrides):
+    """An observation of workload `which` (0: 256^2 / 36,000 visibilities; 4:
+    512^2 / 1,029,000) with the sky and the noise drawn from `seed` (the
+    array layout stays fixed)."""
     from .problem import make_problem
 
     kw = dict(seed_sky=seed, seed_noise=seed + 7919)
     kw.update(overrides)
-    return make_problem(0, **kw)
+    return make_problem(which, **kw)
 
 
 def dirty_image(ctx: Context, p) -> np.ndarray:
@@ -334,12 +335,13 @@ class Pairs:
     seconds: float = 0.0
 
 
-def make_pairs(seeds, iterations: int = 400, optimizer: str = "CG-FRPRMN") -> Pairs:
+def make_pairs(seeds, iterations: int = 400, optimizer: str = "CG-FRPRMN",
+               which: int = 0) -> Pairs:
     """Training pairs (dirty image, MEM solution) from the plain solver."""
     t0 = time.perf_counter()
     dirty, model = [], []
     for s in seeds:
-        p = observation(s)
+        p = observation(s, which)
         ctx = Context(p)
         dirty.append(dirty_image(ctx, p))
         ctx.close()
@@ -362,8 +364,9 @@ class WarmStart:
         return np.maximum(np.abs(dirty).reshape(len(dirty), -1).max(axis=1), 1e-30)
 
     def train(self, pairs: Pairs, steps: int = 400, lr: float = 1e-3, seed: int = 0,
-              log_every: int = 50) -> float:
-        """Adam on the MSE of the scaled model; returns the seconds taken."""
+              log_every: int = 50, progress=None) -> float:
+        """Adam on the MSE of the scaled model; returns the seconds taken.
+        `progress(step, loss)` is called with every logged loss."""
         n = len(pairs.dirty)
         s = self._scale(pairs.dirty)
         ms = np.maximum(np.abs(pairs.model).reshape(n, -1).max(axis=1), 1e-30)
@@ -384,6 +387,8 @@ class WarmStart:
             self.net.adam(lr=lr)
             if step % log_every == 0 or step == steps - 1:
                 self.history.append((step, loss))
+                if progress:
+                    progress(step, loss)
         self.ctx.sync()
         target.free()
         return time.perf_counter() - t0
@@ -417,7 +422,7 @@ def time_to(traj, secs, target: float):
 
 
 def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-FRPRMN",
-             plain: bool = False) -> dict:
+             plain: bool = False, which: int = 0, progress=None) -> dict:
     """The plain solver from the default image and from the warm start on
     held-out observations: iterations and seconds (dirty image and inference
     included for the warm start) to the objective value both reach.  With
@@ -425,7 +430,7 @@ def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-F
     image) to the value all three reach."""
     rows = []
     for s in seeds:
-        p = observation(s)
+        p = observation(s, which)
         ctx = Context(p)
         t0 = time.perf_counter()
         dirty = dirty_image(ctx, p)
@@ -449,6 +454,8 @@ def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-F
                          iters_default=a[0], seconds_default=a[1],
                          iters_warm=b[0], seconds_warm=b[1] + t_pre, warm_overhead_s=t_pre,
                          iters_run_default=len(tr_d), iters_run_warm=len(tr_w), **extra))
+        if progress:
+            progress(rows[-1])
     it_d = np.array([r["iters_default"] for r in rows], float)
     it_w = np.maximum(np.array([r["iters_warm"] for r in rows], float), 1.0)
     t_d = np.array([r["seconds_default"] for r in rows])
