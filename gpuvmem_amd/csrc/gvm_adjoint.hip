@@ -140,25 +140,6 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned int* hi,
   }
 }
 
-// Load through a global-address-space pointer.  Pointers read from device
-// tables are generic, and generic (flat) loads also count against the LDS
-// wait counter, so every LDS wait would wait for them too.
-// (HIP's float2 / float4 are classes whose copy goes through a generic
-// reference, so they are loaded as native vectors.)
-typedef float gf2_t __attribute__((ext_vector_type(2)));
-typedef float gf4_t __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ float ldg(const float* p) {
-  return *(const __attribute__((address_space(1))) float*)p;
-}
-__device__ __forceinline__ float2 ldg(const float2* p) {
-  const gf2_t v = *(const __attribute__((address_space(1))) gf2_t*)p;
-  return make_float2(v.x, v.y);
-}
-__device__ __forceinline__ float4 ldg(const float4* p) {
-  const gf4_t v = *(const __attribute__((address_space(1))) gf4_t*)p;
-  return make_float4(v.x, v.y, v.z, v.w);
-}
-
 // Workgroup barrier that orders LDS accesses only: global loads issued
 // before it (prefetches into registers) stay in flight across it, where
 // __syncthreads would wait for them.

@@ -280,35 +280,59 @@ __global__ __launch_bounds__(256) void degrid_chi2_kernel(
   double acc = 0.0;
   float cm = 0.0f;
   const int mask = N - 1;
-  for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < nvis;
-       v += (long)gridDim.x * blockDim.x) {
-    const int c = cell[v];
-    float2 m = make_float2(0.f, 0.f);
-    if (c >= 0) {
-      const int i1 = c & 0xffff, j1 = c >> 16;
-      const int i2 = (i1 + 1) & mask, j2 = (j1 + 1) & mask;
-      const float2 f = frac[v];
-      const float2 v11 = grid[(long)N * j1 + i1];
-      const float2 v12 = grid[(long)N * j2 + i1];
-      const float2 v21 = grid[(long)N * j1 + i2];
-      const float2 v22 = grid[(long)N * j2 + i2];
-      const float w11 = (1.0f - f.x) * (1.0f - f.y);
-      const float w12 = (1.0f - f.x) * f.y;
-      const float w21 = f.x * (1.0f - f.y);
-      const float w22 = f.x * f.y;
-      m.x = w11 * v11.x + w12 * v12.x + w21 * v21.x + w22 * v22.x;
-      m.y = w11 * v11.y + w12 * v12.y + w21 * v21.y + w22 * v22.y;
+  // Grid-stride loop, four strides per pass: the loads of four visibilities
+  // are issued before their arithmetic, and they are accumulated in stride
+  // order, so the per-thread sums are those of the one-at-a-time loop.
+  const long S = (long)gridDim.x * blockDim.x;
+  for (long v0 = blockIdx.x * (long)blockDim.x + threadIdx.x; v0 < nvis; v0 += 4 * S) {
+    int c[4];
+    float2 f[4], o[4];
+    float w[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const long v = v0 + u * S;
+      const bool ok = v < nvis;
+      c[u] = ok ? ldg(cell + v) : -1;
+      f[u] = ok ? ldg(frac + v) : make_float2(0.f, 0.f);
+      o[u] = ok ? ldg(vo + v) : make_float2(0.f, 0.f);
+      w[u] = ok ? ldg(weight + v) : 0.0f;
     }
-    const float2 o = vo[v];
-    const float2 r = make_float2(o.x - m.x, o.y - m.y);
-    vr[v] = r;
-    const float w = weight[v];
-    const float a2 = r.x * r.x + r.y * r.y;
-    acc += (double)(w * a2);
-    const float amp = w * sqrtf(a2);
-    cm = fmaxf(cm, amp);
-    // c = w conj(Vr) in polar form for the adjoint: (|c|, arg(c) in turns)
-    cpol[v] = make_float2(amp, -atan2f(r.y, r.x) * 0.15915494309189535f);
+    float2 t11[4], t12[4], t21[4], t22[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      t11[u] = t12[u] = t21[u] = t22[u] = make_float2(0.f, 0.f);
+      if (c[u] >= 0) {
+        const int i1 = c[u] & 0xffff, j1 = c[u] >> 16;
+        const int i2 = (i1 + 1) & mask, j2 = (j1 + 1) & mask;
+        t11[u] = grid[(long)N * j1 + i1];
+        t12[u] = grid[(long)N * j2 + i1];
+        t21[u] = grid[(long)N * j1 + i2];
+        t22[u] = grid[(long)N * j2 + i2];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const long v = v0 + u * S;
+      if (v >= nvis) break;
+      float2 m = make_float2(0.f, 0.f);
+      if (c[u] >= 0) {
+        const float2 fu = f[u];
+        const float w11 = (1.0f - fu.x) * (1.0f - fu.y);
+        const float w12 = (1.0f - fu.x) * fu.y;
+        const float w21 = fu.x * (1.0f - fu.y);
+        const float w22 = fu.x * fu.y;
+        m.x = w11 * t11[u].x + w12 * t12[u].x + w21 * t21[u].x + w22 * t22[u].x;
+        m.y = w11 * t11[u].y + w12 * t12[u].y + w21 * t21[u].y + w22 * t22[u].y;
+      }
+      const float2 r = make_float2(o[u].x - m.x, o[u].y - m.y);
+      stg(vr + v, r);
+      const float a2 = r.x * r.x + r.y * r.y;
+      acc += (double)(w[u] * a2);
+      const float amp = w[u] * sqrtf(a2);
+      cm = fmaxf(cm, amp);
+      // c = w conj(Vr) in polar form for the adjoint: (|c|, arg(c) in turns)
+      stg(cpol + v, make_float2(amp, -atan2f(r.y, r.x) * 0.15915494309189535f));
+    }
   }
   const double tot = block_sum(acc, sh);
   __syncthreads();

@@ -153,6 +153,34 @@ gvm_status primary_beam_dev(const gvm_geometry& g, double freq, float* out, hipS
 gvm_status read_scalars(gvm_ctx* ctx, int nslots, double* out);
 int reduce_grid(long n);
 
+// Loads and stores through global-address-space pointers.  Pointers read
+// from device tables (the slice descriptors) are generic, and generic (flat)
+// accesses also count against the LDS wait counter, so every LDS wait would
+// wait for them too.  HIP's float2 / float4 are classes whose copy goes
+// through a generic reference, so they are moved as native vectors.
+typedef float gf2_t __attribute__((ext_vector_type(2)));
+typedef float gf4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float ldg(const float* p) {
+  return *(const __attribute__((address_space(1))) float*)p;
+}
+__device__ __forceinline__ int ldg(const int* p) {
+  return *(const __attribute__((address_space(1))) int*)p;
+}
+__device__ __forceinline__ float2 ldg(const float2* p) {
+  const gf2_t v = *(const __attribute__((address_space(1))) gf2_t*)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 ldg(const float4* p) {
+  const gf4_t v = *(const __attribute__((address_space(1))) gf4_t*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stg(float2* p, float2 v) {
+  gf2_t w;
+  w.x = v.x;
+  w.y = v.y;
+  *(__attribute__((address_space(1))) gf2_t*)p = w;
+}
+
 // Wave-level (64 lanes) and block-level sums in double.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
