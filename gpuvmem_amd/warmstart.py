@@ -425,9 +425,10 @@ def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-F
              plain: bool = False, which: int = 0, progress=None) -> dict:
     """The plain solver from the default image and from the warm start on
     held-out observations: iterations and seconds (dirty image and inference
-    included for the warm start) to the objective value both reach.  With
-    plain=True, also the plain FP32 form (FP32 adjoint, no graphs, default
-    image) to the value all three reach."""
+    included for the warm start) to the objective value both reach, and the
+    relative L2 error of both final images against the true sky inside the
+    mask.  With plain=True, also the plain FP32 form (FP32 adjoint, no
+    graphs, default image) to the value all three reach."""
     rows = []
     for s in seeds:
         p = observation(s, which)
@@ -437,23 +438,32 @@ def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-F
         init = ws.predict(dirty[None])[0]
         t_pre = time.perf_counter() - t0
         ctx.close()
-        _, tr_d, sc_d, _ = solve(p, iterations=iterations, optimizer=optimizer)
+        img_d, tr_d, sc_d, _ = solve(p, iterations=iterations, optimizer=optimizer)
         img_w, tr_w, sc_w, _ = solve(p, init=init, iterations=iterations, optimizer=optimizer)
+        # final images against the true sky (internal units) inside the mask
+        mask = p.noise < p.noise_cut
+        truth = (p.sky / p.fg_scale)[mask]
+        err_d = float(np.linalg.norm(img_d[mask] - truth) / np.linalg.norm(truth))
+        err_w = float(np.linalg.norm(img_w[mask] - truth) / np.linalg.norm(truth))
+        diff = float(np.linalg.norm(img_w[mask] - img_d[mask]) / np.linalg.norm(img_d[mask]))
         target = max(tr_d[-1], tr_w[-1])
         extra = {}
         if plain:
-            _, tr_p, sc_p, _ = solve(p, iterations=iterations, optimizer=optimizer, graphs=False,
-                                     adjoint="f32")
+            img_p, tr_p, sc_p, _ = solve(p, iterations=iterations, optimizer=optimizer,
+                                         graphs=False, adjoint="f32")
             t3 = max(target, tr_p[-1])
             cp, cw = time_to(tr_p, sc_p, t3), time_to(tr_w, sc_w, t3)
             extra = dict(f_plain=tr_p[-1], iters_plain=cp[0], seconds_plain=cp[1],
-                         seconds_warm_to_common=cw[1] + t_pre)
+                         seconds_warm_to_common=cw[1] + t_pre,
+                         sky_err_plain=float(np.linalg.norm(img_p[mask] - truth) /
+                                             np.linalg.norm(truth)))
         a, b = time_to(tr_d, sc_d, target), time_to(tr_w, sc_w, target)
         rows.append(dict(seed=s, f_default=tr_d[-1], f_warm=tr_w[-1], f0_default=tr_d[0],
                          f0_warm=tr_w[0],
                          iters_default=a[0], seconds_default=a[1],
                          iters_warm=b[0], seconds_warm=b[1] + t_pre, warm_overhead_s=t_pre,
-                         iters_run_default=len(tr_d), iters_run_warm=len(tr_w), **extra))
+                         iters_run_default=len(tr_d), iters_run_warm=len(tr_w),
+                         sky_err_default=err_d, sky_err_warm=err_w, image_rel_diff=diff, **extra))
         if progress:
             progress(rows[-1])
     it_d = np.array([r["iters_default"] for r in rows], float)
@@ -462,8 +472,12 @@ def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-F
     t_w = np.array([r["seconds_warm"] for r in rows])
     out = {}
     if plain:
+        out["median_sky_err_plain"] = float(np.median([r["sky_err_plain"] for r in rows]))
         out["median_time_ratio_plain_fp32"] = float(np.median(
             [r["seconds_plain"] / r["seconds_warm_to_common"] for r in rows]))
+    out["median_sky_err_default"] = float(np.median([r["sky_err_default"] for r in rows]))
+    out["median_sky_err_warm"] = float(np.median([r["sky_err_warm"] for r in rows]))
+    out["median_image_rel_diff"] = float(np.median([r["image_rel_diff"] for r in rows]))
     return dict(rows=rows, **out, median_iter_ratio=float(np.median(it_d / it_w)),
                 median_time_ratio=float(np.median(t_d / t_w)),
                 mean_iters_default=float(it_d.mean()), mean_iters_warm=float(it_w.mean()),
