@@ -144,9 +144,11 @@ def test_setup_matches_oracle_restatement():
 
 def test_generator_counts_and_determinism():
     fw = native.load_fw()
-    for which, n in ((0, 36000), (1, 1764000)):
+    for which, n, npix in ((0, 36000, 256), (1, 1764000, 1024), (2, 4354560, 2048),
+                           (3, 299880, 512), (4, 1029000, 512)):
         cfg = sim_config(which)
         assert fw.gvm_sim_count_visibilities(C.byref(cfg)) == n
+        assert cfg.npix == npix
     p1 = small_problem(seed=5)
     p2 = small_problem(seed=5)
     assert np.array_equal(p1.uvw, p2.uvw) and np.array_equal(p1.vo, p2.vo)
