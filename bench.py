@@ -179,27 +179,49 @@ def bench_gridding(p, steps: int, warmup: int, threads: int, adjoint: str) -> di
     return out
 
 
+def generate_pairs(seeds, iterations: int, which: int):
+    """Training pairs from the plain solver for `seeds`, in groups of 16 with
+    a progress line after each group."""
+    import numpy as np
+    from gpuvmem_amd import warmstart as W
+
+    t0 = time.perf_counter()
+    seeds = list(seeds)
+    chunks = []
+    for c0 in range(0, len(seeds), 16):
+        chunks.append(W.make_pairs(seeds[c0:c0 + 16], iterations=iterations, which=which))
+        print(f"# pairs {c0 + len(chunks[-1].dirty)} ({time.perf_counter() - t0:.1f} s)",
+              file=sys.stderr, flush=True)
+    return W.Pairs(np.concatenate([c.dirty for c in chunks]),
+                   np.concatenate([c.model for c in chunks]), seeds,
+                   time.perf_counter() - t0)
+
+
 def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, levels: int,
-                     base: int, which: int = 0, batch: int = 8) -> dict:
+                     base: int, which: int = 0, batch: int = 8, load=None) -> dict:
     """The learned warm start on observations of workload `which` (0: 256^2 /
     36,000 visibilities; 4: 512^2 / 1,029,000): training pairs from the plain
-    solver (CG-FRPRMN to its tolerance), U-Net training on the device kernels,
-    then the plain solver from the default image and from the prediction on
-    held-out observations."""
+    solver (CG-FRPRMN to its tolerance; generated here, or loaded from the
+    files `load` that --ws-save runs wrote), U-Net training on the device
+    kernels, then the plain solver from the default image and from the
+    prediction on held-out observations."""
     import numpy as np
     from gpuvmem_amd import warmstart as W
     from gpuvmem_amd.device import Context
 
     t0 = time.perf_counter()
-    chunks = []
-    for c0 in range(0, n_train, 16):
-        chunks.append(W.make_pairs(range(1000 + c0, 1000 + min(n_train, c0 + 16)),
-                                   iterations=iterations, which=which))
-        print(f"# pairs {c0 + len(chunks[-1].dirty)} ({time.perf_counter() - t0:.1f} s)",
-              file=sys.stderr, flush=True)
-    pairs = W.Pairs(np.concatenate([c.dirty for c in chunks]),
-                    np.concatenate([c.model for c in chunks]))
-    t_pairs = time.perf_counter() - t0
+    if load:
+        parts = [np.load(f) for f in load]
+        pairs = W.Pairs(np.concatenate([q["dirty"] for q in parts]),
+                        np.concatenate([q["model"] for q in parts]),
+                        [int(x) for q in parts for x in q["seeds"]])
+        # generated by separate processes side by side: the longest one
+        t_pairs = max(float(q["seconds"]) for q in parts)
+        n_train = len(pairs.dirty)
+        print(f"# {n_train} pairs from {len(parts)} files", file=sys.stderr, flush=True)
+    else:
+        pairs = generate_pairs(range(1000, 1000 + n_train), iterations, which)
+        t_pairs = pairs.seconds
     p = W.observation(1000, which)
     ctx = Context(p, add_data=False)
     ws = W.WarmStart(ctx, levels=levels, base=base, batch=batch, npix=p.M)
@@ -222,7 +244,8 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
                    "unet": {"levels": levels, "base_channels": base, "parameters": ws.net.nparams,
                             "weights": "bf16 (fp32 master, fp32 accumulation)" if ws.net.bf16
                             else "fp32", "batch": batch},
-                   "train_pairs": n_train, "adam_steps": steps, "held_out": n_test},
+                   "train_pairs": n_train, "adam_steps": steps, "held_out": n_test,
+                   "pair_files": len(load) if load else 0},
         "pairs_seconds": t_pairs, "train_seconds": t_train,
         "train_loss": ws.history, **r,
         "rows": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in row.items()}
@@ -415,6 +438,12 @@ def main() -> int:
     ap.add_argument("--ws-base", type=int, default=None,
                     help="warm start: U-Net base channels (default 16 at 256^2, 32 at 512^2)")
     ap.add_argument("--ws-batch", type=int, default=8, help="warm start: training batch")
+    ap.add_argument("--ws-save", metavar="FILE",
+                    help="warm start: only generate the pairs of --ws-seeds and save them (.npz)")
+    ap.add_argument("--ws-seeds", metavar="A:B", default=None,
+                    help="warm start: sky seeds [A, B) of the pairs that --ws-save generates")
+    ap.add_argument("--ws-load", nargs="+", metavar="FILE",
+                    help="warm start: train on the pairs of these files instead of generating")
     ap.add_argument("--gridding", action="store_true",
                     help="time the device weighting and gridder and the gridded objective step")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -433,12 +462,23 @@ def main() -> int:
             from gpuvmem_amd import warmstart
 
             warmstart.GRAPHS = False
+        if args.ws_save:
+            import numpy as np
+
+            a, b = (int(x) for x in (args.ws_seeds or "1000:%d" % (1000 + args.ws_pairs)).split(":"))
+            pr = generate_pairs(range(a, b), 500, args.ws_workload)
+            with open(args.ws_save, "wb") as f:
+                np.savez(f, dirty=pr.dirty, model=pr.model, seeds=np.array(pr.seeds),
+                         seconds=pr.seconds)
+            print(json.dumps({"saved": args.ws_save, "pairs": len(pr.dirty),
+                              "seconds": pr.seconds}), flush=True)
+            return 0
         big = args.ws_workload == 4
         r = bench_warm_start(n_train=args.ws_pairs, n_test=args.ws_test, steps=args.ws_steps,
                              iterations=500,
                              levels=args.ws_levels or (6 if big else 4),
                              base=args.ws_base or (32 if big else 16),
-                             which=args.ws_workload, batch=args.ws_batch)
+                             which=args.ws_workload, batch=args.ws_batch, load=args.ws_load)
         print(json.dumps(r), flush=True)
         return 0
     t0 = time.perf_counter()
