@@ -1,7 +1,8 @@
 """GPU parity at other sizes and shapes: small images (tile larger than the
-image), several channels with a spectral index (both chain-rule branches), the
-schedule clip, normalisation, off-grid samples, and the full-size workload
-(1024^2, 1,764,000 visibilities) checked on pixel subsets and by
+image), the largest transform (4096^2), one visibility, several channels with
+a spectral index (both chain-rule branches), the schedule clip,
+normalisation, off-grid samples (some, and all of them), and the full-size
+workload (1024^2, 1,764,000 visibilities) checked on pixel subsets and by
 size-independent properties."""
 import numpy as np
 import pytest
@@ -74,6 +75,52 @@ def test_off_grid_samples_get_zero_weight():
     vr = ctx.residuals()
     # off-grid: Vm = 0 so Vr = Vo
     assert np.array_equal(vr[::7], p.vo[::7])
+    ctx.close()
+
+
+def test_single_visibility():
+    p = small_problem(npix=64, n_antennas=2, dump_seconds=3600.0)  # 1 baseline x 1 dump
+    print(f"case: one visibility, {p.M}^2", flush=True)
+    assert p.nvis == 1
+    # one sample gives a degenerate synthesized beam (zero fg_scale and noise cut, an
+    # empty mask), as in the reference's set-up; give the image a mask and a scale
+    assert p.fg_scale == 0.0 and not np.any(p.noise < p.noise_cut)
+    p.noise = np.zeros_like(p.noise)
+    p.fg_scale, p.noise_cut = 1e-7, 1e-6
+    for mode in ("bf16x3", "f32"):
+        ctx = Context(p, adjoint=mode)
+        _check(ctx, oracle.Oracle(p), _rand_img(p, 4))
+        ctx.close()
+
+
+def test_every_sample_off_the_grid():
+    # every weight is zeroed by the off-grid rule, so chi2 and its gradient are 0
+    p = small_problem(npix=64, n_antennas=5)
+    print(f"case: all off grid, {p.M}^2 nvis={p.nvis}", flush=True)
+    p.uvw = p.uvw.copy()
+    dxr = p.deltax * np.pi / 180.0
+    p.uvw[:, 0] = -3.7 * p.M / (p.M * dxr) * np.sign(dxr)
+    orc = oracle.Oracle(p)
+    img = _rand_img(p, 5)
+    ref, vr, clipped = orc.chi2(img)
+    assert ref == 0.0 and np.all(orc.weight == 0)
+    ctx = Context(p)
+    d = ctx.image(img)
+    assert ctx.chi2(d) == 0.0
+    res = ctx.zeros(2 * p.M * p.N)
+    ctx.dchi2(d, res)
+    ctx.sync()
+    assert np.all(res.download(np.float32) == 0.0)
+    assert np.array_equal(ctx.residuals(), p.vo)
+    ctx.close()
+
+
+def test_largest_image():
+    # 4096^2, the largest transform size, with a few dozen visibilities
+    p = small_problem(npix=4096, n_antennas=4, dump_seconds=600.0, seed=7)
+    print(f"case: {p.M}^2 nvis={p.nvis}", flush=True)
+    ctx = Context(p)
+    _check(ctx, oracle.Oracle(p), _rand_img(p, 6))
     ctx.close()
 
 
