@@ -133,9 +133,13 @@ class UNet:
         native.check(st, what)
 
     # ---- forward ----
-    def forward(self) -> None:
-        """self.x -> self.y for the whole batch."""
-        lib, c, B = self.lib, self.ctx.ctx, self.B
+    def forward(self, nb: Optional[int] = None) -> None:
+        """self.x -> self.y for the first `nb` images of the batch (default:
+        all of them); the images are independent, so fewer cost less."""
+        lib, c = self.lib, self.ctx.ctx
+        B = self.B if nb is None else int(nb)
+        if not 1 <= B <= self.B:
+            raise ValueError(f"forward: {B} images for a batch of {self.B}")
         L, ch = self.levels, self.ch
         cin = 1
         for l in range(L):
@@ -234,9 +238,12 @@ class UNet:
     def set_input(self, xb: np.ndarray) -> None:
         self.x.upload(np.ascontiguousarray(xb, np.float32))
 
-    def output(self) -> np.ndarray:
+    def output(self, nb: Optional[int] = None) -> np.ndarray:
+        """The first `nb` outputs (default: the whole batch), (nb, 1, H, W)."""
         self.ctx.sync()
-        return self.y.download(np.float32, (self.B, 1, self.H, self.W))
+        nb = self.B if nb is None else int(nb)
+        n = nb * self.H * self.W
+        return self.y.download(np.float32)[:n].reshape(nb, 1, self.H, self.W)
 
     def params_dict(self) -> dict:
         """The parameters as the convolutions use them (BF16-rounded weights
@@ -400,12 +407,10 @@ class WarmStart:
         s = self._scale(dirty)
         x = (dirty / s[:, None, None]).astype(np.float32)
         for k in range(0, len(x), B):
-            xb = np.zeros((B, 1, self.net.H, self.net.W), np.float32)
-            xb[: len(x[k:k + B]), 0] = x[k:k + B]
-            self.net.set_input(xb)
-            self.net.forward()
-            y = self.net.output()[:, 0]
-            out.extend(y[: len(x[k:k + B])])
+            xk = x[k:k + B]
+            self.net.set_input(xk[:, None])
+            self.net.forward(len(xk))
+            out.extend(self.net.output(len(xk))[:, 0])
         return np.stack(out) * (s * self.calib)[:, None, None]
 
     def close(self) -> None:

@@ -189,4 +189,9 @@ def test_unet_forward_backward_match_oracle(ctx, bf16):
     g1 = net.grads.download(np.float32)
     net.forward(); net.loss(_dev(t)); net.backward()
     assert np.array_equal(net.grads.download(np.float32), g1)
+    # inference on fewer images than the batch gives the same outputs
+    y_all = net.output()
+    net.set_input(x[1:2])
+    net.forward(1)
+    assert np.array_equal(net.output(1)[0], y_all[1])
     net.close()
