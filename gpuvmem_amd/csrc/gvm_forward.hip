@@ -264,10 +264,12 @@ gvm_status model_grids(gvm_ctx* ctx, const float* I, int nsl) {
 // src/functions.cu:2557-2609, 2663-2671, 2867-2876.  Bytes per visibility:
 // cell 4 + frac 8 + Vo 8 + weight 4 read, Vr 8 + polar c 8 written.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void degrid_chi2_kernel(
+constexpr int kDegridThreads = 512;
+
+__global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
     const float2* __restrict__ grids, int N, const SliceDev* __restrict__ tab,
     double* __restrict__ partials, unsigned int* __restrict__ cmaxs) {
-  __shared__ double sh[4];
+  __shared__ double sh[kDegridThreads / 64];
   const SliceDev& sd = tab[blockIdx.y];
   const float2* __restrict__ grid = grids + (long)blockIdx.y * N * N;
   const int* __restrict__ cell = sd.cell;
@@ -373,11 +375,13 @@ gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normal
   GVM_CHECK_LAUNCH();
   gvm_status st = model_grids(ctx, I, nsl);
   if (st) return st;
-  // blocks per slice: 256 visibilities each, at most kReduceBlocks
+  // blocks per slice: one visibility per thread, at most kReduceBlocks
   long vmax = 0;
   for (const Slice& sl : ctx->slices) vmax = std::max(vmax, sl.nvis);
-  const int nb = (int)std::min<long>(kReduceBlocks, std::max<long>(1, (vmax + 255) / 256));
-  hipLaunchKernelGGL(degrid_chi2_kernel, dim3(nb, nsl), dim3(256), 0, ctx->stream, ctx->grid,
+  const int nb = (int)std::min<long>(
+      kReduceBlocks, std::max<long>(1, (vmax + kDegridThreads - 1) / kDegridThreads));
+  hipLaunchKernelGGL(degrid_chi2_kernel, dim3(nb, nsl), dim3(kDegridThreads), 0, ctx->stream,
+                     ctx->grid,
                      g.N, ctx->slice_tab, ctx->partials, ctx->cmax);
   GVM_CHECK_LAUNCH();
   st = finish_reductions(ctx, nsl, nb);
