@@ -57,7 +57,9 @@ typedef struct {
  * 3 = one channel of the 512^2 multi-frequency workload (35 antennas,
  * 595 baselines x 504 dumps = 299,880 rows per channel, nu0 = 225 GHz,
  * spectral indices U(-1, 3); the caller sets freq_hz, flag_fraction and the
- * per-channel seeds). */
+ * per-channel seeds), 4 = the 512^2 / 1,029,000 vis warm-start workload (the
+ * 256^2 skies scaled up, 50 antennas in a 1 km disk, 1225 baselines x 840
+ * dumps). */
 void gvm_sim_default_config(int which, gvm_sim_config* cfg);
 long gvm_sim_count_visibilities(const gvm_sim_config* cfg);
 /* uvw_m: 3*nvis doubles (metres); vo: 2*nvis floats; weight: nvis floats;
