@@ -244,6 +244,38 @@ __global__ __launch_bounds__(256) void grad_condition_kernel(const float* __rest
   if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
+// Device-side scalars of the L-BFGS two-loop recursion.  Each double
+// operation is the one the host form performs, rounded once (no contraction),
+// so the recursion gives the same coefficients on either side.
+__global__ void rho_times_kernel(const double* __restrict__ rho_den,
+                                 const double* __restrict__ dot,
+                                 const double* __restrict__ minus_from, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double rho = *rho_den != 0.0 ? 1.0 / *rho_den : 0.0;
+  const double r = rho * *dot;
+  *out = minus_from ? *minus_from - r : r;
+}
+__global__ void h0_scale_kernel(const double* __restrict__ sy, const double* __restrict__ yy,
+                                int nimg, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double acc = 0.0;
+  for (int i = 0; i < nimg; i++)
+    if (yy[i] != 0.0) acc += sy[i] / yy[i];
+  *out = acc;
+}
+__global__ void axpy_dev_kernel(float* __restrict__ y, const float* __restrict__ x,
+                                const double* __restrict__ coef, int negate, long n) {
+  const float a = (float)(negate ? -*coef : *coef);
+  GRID_LOOP(k, n) y[k] += a * x[k];
+}
+__global__ void scale_dev_kernel(float* y, const float* x, const double* __restrict__ coef,
+                                 long n) {
+  const float a = (float)*coef;
+  GRID_LOOP(k, n) y[k] = x[k] * a;
+}
+
 #undef GRID_LOOP
 
 }  // namespace gvm
@@ -419,6 +451,48 @@ extern "C" gvm_status gvm_vec_axpy(gvm_ctx* ctx, float* y, const float* x, doubl
   GVM_REQUIRE(ctx && y && x && n >= 0, "gvm_vec_axpy: bad argument");
   if (n == 0) return GVM_OK;
   LAUNCH_1D(axpy_kernel, n, y, x, (float)alpha, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_vec_dot_dev(gvm_ctx* ctx, const float* a, const float* b, long n,
+                                      double* out_dev) {
+  GVM_REQUIRE(ctx && a && b && out_dev && n > 0, "gvm_vec_dot_dev: bad argument");
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, ctx->stream, a, b, n, ctx->partials);
+  GVM_CHECK_LAUNCH();
+  return finish_reductions_to(ctx, 1, nb, out_dev);
+}
+
+extern "C" gvm_status gvm_scalar_rho_times(gvm_ctx* ctx, const double* rho_den, const double* dot,
+                                          const double* minus_from, double* out) {
+  GVM_REQUIRE(ctx && rho_den && dot && out, "gvm_scalar_rho_times: null argument");
+  hipLaunchKernelGGL(rho_times_kernel, dim3(1), dim3(64), 0, ctx->stream, rho_den, dot,
+                     minus_from, out);
+  GVM_CHECK_LAUNCH();
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_scalar_h0(gvm_ctx* ctx, const double* sy, const double* yy, int nimg,
+                                   double* out) {
+  GVM_REQUIRE(ctx && sy && yy && out && nimg >= 1, "gvm_scalar_h0: bad argument");
+  hipLaunchKernelGGL(h0_scale_kernel, dim3(1), dim3(64), 0, ctx->stream, sy, yy, nimg, out);
+  GVM_CHECK_LAUNCH();
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_vec_axpy_dev(gvm_ctx* ctx, float* y, const float* x, const double* coef,
+                                      int negate, long n) {
+  GVM_REQUIRE(ctx && y && x && coef && n >= 0, "gvm_vec_axpy_dev: bad argument");
+  if (n == 0) return GVM_OK;
+  LAUNCH_1D(axpy_dev_kernel, n, y, x, coef, negate, n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_vec_scale_dev(gvm_ctx* ctx, float* dst, const float* src,
+                                       const double* coef, long n) {
+  GVM_REQUIRE(ctx && dst && src && coef && n >= 0, "gvm_vec_scale_dev: bad argument");
+  if (n == 0) return GVM_OK;
+  LAUNCH_1D(scale_dev_kernel, n, dst, src, coef, n);
   return GVM_OK;
 }
 

@@ -19,6 +19,7 @@ struct gvm_fw_objective {
   std::vector<std::unique_ptr<Fi>> terms;
   std::unique_ptr<ObjectiveFunction> of;
   imageMap* mapping = nullptr;
+  bool lbfgs_host_scalars = false;
   ~gvm_fw_objective() { delete[] mapping; }
 };
 
@@ -142,6 +143,7 @@ extern "C" gvm_status gvm_fw_optimize_timed(gvm_fw_objective* obj, const char* o
     opt->setFTol((float)ftol);
     opt->setGTol((float)gtol);
     if (lbfgs_k > 0) opt->setK(lbfgs_k);
+    if (auto* lb = dynamic_cast<LBFGS*>(opt.get())) lb->setHostScalars(obj->lbfgs_host_scalars);
     opt->setImage(&image);
     opt->setFlag(obj->cfg.flag_opt);
     const long f0 = obj->of->getFunctionCalls(), g0 = obj->of->getGradientCalls();
@@ -184,6 +186,15 @@ extern "C" gvm_status gvm_fw_set_graphs(gvm_fw_objective* obj, int on) {
     return GVM_ERR_ARG;
   }
   obj->of->setGraphs(on != 0);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_set_lbfgs_host_scalars(gvm_fw_objective* obj, int on) {
+  if (!obj) {
+    g_fw_error = "gvm_fw_set_lbfgs_host_scalars: null objective";
+    return GVM_ERR_ARG;
+  }
+  obj->lbfgs_host_scalars = on != 0;
   return GVM_OK;
 }
 

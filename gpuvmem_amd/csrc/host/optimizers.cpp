@@ -339,6 +339,11 @@ void LBFGS::allocateMemoryGpu() {
   xi_old = dev_zeros(tot);
   d_q = dev_zeros(tot);
   d_r = dev_zeros(tot);
+  // alpha for every (image, memory slot), then the scratch scalars
+  void* sc = nullptr;
+  check(gvm_malloc(&sc, sizeof(double) * ((size_t)image->getImageCount() * K + 16)),
+        "gvm_malloc(lbfgs scalars)");
+  d_scal = (double*)sc;
   linmin_alloc(tot);
 }
 
@@ -350,7 +355,9 @@ void LBFGS::deallocateMemoryGpu() {
   gvm_free(xi_old);
   gvm_free(d_q);
   gvm_free(d_r);
+  gvm_free(d_scal);
   d_y = d_s = p_old = xi = xi_old = d_q = d_r = nullptr;
+  d_scal = nullptr;
   linmin_free();
 }
 
@@ -409,16 +416,30 @@ void LBFGS::optimize() {
   finish(0);
 }
 
+// Two-loop recursion over the stored pairs (src/lbfgs.cu:252-341), with the
+// reference's slot addressing: dot products read pair k of image im at
+// offset n (im k + im), the updates at n (im + k).
 void LBFGS::LBFGS_recursion(float* d_y, float* d_s, float* xi, int par_M, int lbfgs_it, int M,
                             int N) {
+  (void)d_y;
+  (void)d_s;
+  (void)xi;
   const long n = (long)M * N;
   const int nimg = image->getImageCount();
-  const long tot = n * nimg;
+  check(gvm_vec_copy(ctx(), d_q, this->xi, n * nimg), "copy q");
+  check(gvm_vec_zero(ctx(), d_r, n * nimg), "zero r");
+  if (host_scalars)
+    recursion_host(par_M, lbfgs_it, n, nimg);
+  else
+    recursion_device(par_M, lbfgs_it, n, nimg);
+  check(gvm_vec_scale(ctx(), this->xi, d_r, -1.0, n * nimg), "searchDirection_LBFGS(r)");
+}
+
+// Every scalar read back to the host, as the reference does.
+void LBFGS::recursion_host(int par_M, int lbfgs_it, long n, int nimg) {
   std::vector<double> alpha((size_t)nimg * par_M, 0.0);
   auto blk_dot = [&](float* buf, int im, int k) { return buf + n * ((long)im * k + im); };
   auto blk_upd = [&](float* buf, int im, int k) { return buf + n * ((long)im + k); };
-  check(gvm_vec_copy(ctx(), d_q, xi, tot), "copy q");
-  check(gvm_vec_zero(ctx(), d_r, tot), "zero r");
   for (int im = 0; im < nimg; im++) {
     for (int k = par_M - 1; k >= 0; k--) {
       double rho_den = 0.0, sq = 0.0;
@@ -449,7 +470,43 @@ void LBFGS::LBFGS_recursion(float* d_y, float* d_s, float* xi, int par_M, int lb
             "updateQ(r)");
     }
   }
-  check(gvm_vec_scale(ctx(), xi, d_r, -1.0, tot), "searchDirection_LBFGS(r)");
+}
+
+// The same operations with the scalars kept in device doubles: the whole
+// recursion is enqueued without a host round trip.
+void LBFGS::recursion_device(int par_M, int lbfgs_it, long n, int nimg) {
+  auto blk_dot = [&](float* buf, int im, int k) { return buf + n * ((long)im * k + im); };
+  auto blk_upd = [&](float* buf, int im, int k) { return buf + n * ((long)im + k); };
+  double* alpha = d_scal;                     // [nimg * par_M]
+  double* t = d_scal + (size_t)nimg * K;      // scratch
+  double *rho_den = t, *dot = t + 1, *coef = t + 2, *sy = t + 3, *yy = t + 3 + nimg,
+         *h0 = t + 3 + 2 * nimg;
+  for (int im = 0; im < nimg; im++) {
+    for (int k = par_M - 1; k >= 0; k--) {
+      double* a = alpha + (size_t)im * par_M + k;
+      check(gvm_vec_dot_dev(ctx(), blk_dot(d_y, im, k), blk_dot(d_s, im, k), n, rho_den), "y.s");
+      check(gvm_vec_dot_dev(ctx(), blk_dot(d_s, im, k), d_q + n * im, n, dot), "s.q");
+      check(gvm_scalar_rho_times(ctx(), rho_den, dot, nullptr, a), "alpha");
+      check(gvm_vec_axpy_dev(ctx(), d_q + n * im, blk_upd(d_y, im, k), a, 1, n), "updateQ");
+    }
+  }
+  for (int im = 0; im < nimg; im++) {
+    check(gvm_vec_dot_dev(ctx(), blk_dot(d_y, im, lbfgs_it), blk_dot(d_s, im, lbfgs_it), n, sy + im),
+          "s.y");
+    check(gvm_vec_dot_dev(ctx(), blk_dot(d_y, im, lbfgs_it), blk_dot(d_y, im, lbfgs_it), n, yy + im),
+          "y.y");
+  }
+  check(gvm_scalar_h0(ctx(), sy, yy, nimg, h0), "H0 scale");
+  for (int im = 0; im < nimg; im++) {
+    check(gvm_vec_scale_dev(ctx(), d_r + n * im, d_q + n * im, h0, n), "getR");
+    for (int k = 0; k < par_M; k++) {
+      check(gvm_vec_dot_dev(ctx(), blk_dot(d_y, im, k), blk_dot(d_s, im, k), n, rho_den), "y.s");
+      check(gvm_vec_dot_dev(ctx(), blk_dot(d_y, im, k), d_r + n * im, n, dot), "y.r");
+      check(gvm_scalar_rho_times(ctx(), rho_den, dot, alpha + (size_t)im * par_M + k, coef),
+            "alpha - beta");
+      check(gvm_vec_axpy_dev(ctx(), d_r + n * im, blk_upd(d_s, im, k), coef, 0, n), "updateQ(r)");
+    }
+  }
 }
 
 namespace {

@@ -314,6 +314,12 @@ class Objective:
         """Replay captured HIP graphs of repeated evaluations."""
         native.check_fw(self.fw.gvm_fw_set_graphs(self.h, int(on)), "gvm_fw_set_graphs")
 
+    def set_lbfgs_host_scalars(self, on: bool) -> None:
+        """CG-LBFGS reads every dot product of its recursion back to the host
+        (the reference's way) instead of keeping the scalars on the device."""
+        native.check_fw(self.fw.gvm_fw_set_lbfgs_host_scalars(self.h, int(on)),
+                        "gvm_fw_set_lbfgs_host_scalars")
+
     def graph_launches(self) -> int:
         n = C.c_long()
         native.check_fw(self.fw.gvm_fw_graph_launches(self.h, C.byref(n)), "gvm_fw_graph_launches")

@@ -189,6 +189,11 @@ HIP_API = {
     "gvm_vec_scale": [_P, _P, _P, _D, _L],
     "gvm_vec_dot": [_P, _P, _P, _L, _DP],
     "gvm_vec_axpy": [_P, _P, _P, _D, _L],
+    "gvm_vec_dot_dev": [_P, _P, _P, _L, _P],
+    "gvm_scalar_rho_times": [_P, _P, _P, _P, _P],
+    "gvm_scalar_h0": [_P, _P, _P, _I, _P],
+    "gvm_vec_axpy_dev": [_P, _P, _P, _P, _I, _L],
+    "gvm_vec_scale_dev": [_P, _P, _P, _P, _L],
     "gvm_lbfgs_s_and_y": [_P, _P, _P, _P, _P, _P, _P, _L],
     "gvm_primary_beam": [_I, _D, _D, _D, _D, _I, _I, _F, _F, _D, _D, _P],
     "gvm_sync": [_P],
@@ -230,6 +235,7 @@ FW_API = {
                                _I], _I),
     "gvm_fw_set_flag": ([_P, _I], _I),
     "gvm_fw_set_graphs": ([_P, _I], _I),
+    "gvm_fw_set_lbfgs_host_scalars": ([_P, _I], _I),
     "gvm_fw_graph_launches": ([_P, C.POINTER(_L)], _I),
     "gvm_fw_last_error": ([], C.c_char_p),
     "gvm_fw_registered": ([C.c_char_p, C.c_char_p], _I),

@@ -84,6 +84,11 @@ gvm_status gvm_fw_set_flag(gvm_fw_objective* obj, int flag_opt);
 gvm_status gvm_fw_set_graphs(gvm_fw_objective* obj, int on);
 /* Graph replays so far (value and gradient evaluations served by a graph). */
 gvm_status gvm_fw_graph_launches(gvm_fw_objective* obj, long* n);
+/* CG-LBFGS's two-loop recursion (src/lbfgs.cu:252-341) reads every dot product
+ * back to the host when on != 0, as the reference does; by default (0) its
+ * scalars stay on the device and the recursion runs without host round
+ * trips.  Both give the same search direction, bit for bit. */
+gvm_status gvm_fw_set_lbfgs_host_scalars(gvm_fw_objective* obj, int on);
 const char* gvm_fw_last_error(void);
 
 /* 1 if `name` is registered with the factory of `kind` ("Fi", "Optimizer",

@@ -26,11 +26,19 @@ class LBFGS : public Optimizer {
   void deallocateMemoryGpu() override;
   void optimize() override;
   void LBFGS_recursion(float* d_y, float* d_s, float* xi, int par_M, int lbfgs_it, int M, int N);
+  // Not in the reference: read every dot product of the recursion back to
+  // the host, as the reference does, instead of keeping the recursion's
+  // scalars on the device (the default; both give the same direction).
+  void setHostScalars(bool on) { host_scalars = on; }
 
  private:
+  void recursion_host(int par_M, int lbfgs_it, long n, int nimg);
+  void recursion_device(int par_M, int lbfgs_it, long n, int nimg);
   double fret = 0, fp = 0;
   float *d_s = nullptr, *d_y = nullptr, *xi = nullptr, *xi_old = nullptr, *p_old = nullptr;
   float *d_q = nullptr, *d_r = nullptr;
+  double* d_scal = nullptr;  // device scalars of the recursion
+  bool host_scalars = false;
   int configured = 1;
   int K = 100;
 };

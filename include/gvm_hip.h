@@ -360,6 +360,27 @@ gvm_status gvm_vec_dot(gvm_ctx* ctx, const float* a, const float* b, long n,
 /* updateQ (src/functions.cu:3614-3621): y += alpha x over n elements. */
 gvm_status gvm_vec_axpy(gvm_ctx* ctx, float* y, const float* x, double alpha,
                         long n);
+/* Enqueue-only forms of the L-BFGS recursion's scalars (no counterpart in the
+ * reference, which reads every dot product back to the host: src/lbfgs.cu:
+ * 270-341): the scalars stay in device doubles (caller-owned device memory)
+ * and the vector updates read them there, so the two-loop recursion runs
+ * without a host round trip.  Each scalar is computed with the host form's
+ * double operations, rounded once each, so the coefficients are identical.
+ *   gvm_vec_dot_dev:      *out_dev = sum a*b (FP64, the order of gvm_vec_dot);
+ *   gvm_scalar_rho_times: r = (*rho_den != 0 ? 1 / *rho_den : 0) * *dot;
+ *                         *out = minus_from ? *minus_from - r : r;
+ *   gvm_scalar_h0:        *out = sum over i < nimg of sy[i] / yy[i] (yy[i] != 0);
+ *   gvm_vec_axpy_dev:     y += (float)(negate ? -*coef : *coef) * x;
+ *   gvm_vec_scale_dev:    dst = src * (float)*coef. */
+gvm_status gvm_vec_dot_dev(gvm_ctx* ctx, const float* a, const float* b, long n, double* out_dev);
+gvm_status gvm_scalar_rho_times(gvm_ctx* ctx, const double* rho_den, const double* dot,
+                                const double* minus_from, double* out);
+gvm_status gvm_scalar_h0(gvm_ctx* ctx, const double* sy, const double* yy, int nimg,
+                         double* out);
+gvm_status gvm_vec_axpy_dev(gvm_ctx* ctx, float* y, const float* x, const double* coef,
+                            int negate, long n);
+gvm_status gvm_vec_scale_dev(gvm_ctx* ctx, float* dst, const float* src, const double* coef,
+                             long n);
 /* calculateSandY (src/functions.cu:3635-3653) for one image:
  * y = xi + xi_old; s = p - p_old over n elements. */
 gvm_status gvm_lbfgs_s_and_y(gvm_ctx* ctx, float* y, float* s, const float* p,

@@ -67,9 +67,10 @@ def test_zero_lambda_terms_are_dropped():
     ctx.close()
 
 
-def _run(p, optimizer, iterations, terms, lambdas, k=0, mode="f16x3"):
+def _run(p, optimizer, iterations, terms, lambdas, k=0, mode="f16x3", host_scalars=False):
     ctx = Context(p, adjoint=mode)
     obj = Objective(ctx, fw_config(terms, lambdas=lambdas))
+    obj.set_lbfgs_host_scalars(host_scalars)
     d = ctx.image(p.initial_image())
     res, traj = obj.optimize(d, optimizer, iterations, lbfgs_k=k)
     img = d.download(np.float32, (2, p.N, p.M))
@@ -137,6 +138,21 @@ def test_lbfgs_on_device_is_memory_one_for_image0():
     assert np.all(terr < 1e-4)
     assert np.all(np.diff(t10) <= 0)
     assert abs(r10.final_value - ores.final_value) <= 0.2 * abs(ores.final_value)
+
+
+def test_lbfgs_device_scalars_match_host_scalars():
+    """The two-loop recursion with its scalars kept on the device gives the
+    same directions as the form that reads every dot product back to the
+    host: identical trajectories and images, for one and two images."""
+    p = small_problem(npix=128, n_antennas=10, seed=22)
+    for terms, lam in (([("Chi2", -1, 0.0), ("Entropy", 0, 0.001)], (0.01,)),
+                       ([("Chi2", -1, 0.0), ("Entropy", 0, 0.001),
+                         ("TotalSquaredVariation", 1, 0.0)], (0.01, 1e-3))):
+        print(f"case: L-BFGS scalars, {len(terms)} terms", flush=True)
+        a = _run(p, "CG-LBFGS", 12, terms, lam, k=10, host_scalars=True)
+        b = _run(p, "CG-LBFGS", 12, terms, lam, k=10, host_scalars=False)
+        assert len(a[2]) > 3
+        assert np.array_equal(a[2], b[2]) and np.array_equal(a[0], b[0])
 
 
 def test_mfs_flag_schedule_two_channels():
