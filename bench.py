@@ -200,7 +200,7 @@ def generate_pairs(seeds, iterations: int, which: int):
 def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, levels: int,
                      base: int, which: int = 0, batch: int = 8, load=None) -> dict:
     """The learned warm start on observations of workload `which` (0: 256^2 /
-    36,000 visibilities; 4: 512^2 / 1,029,000): training pairs from the plain
+    36,000 visibilities; 1: 1024^2 / 1,764,000; 4: 512^2 / 1,029,000): training pairs from the plain
     solver (CG-FRPRMN to its tolerance; generated here, or loaded from the
     files `load` that --ws-save runs wrote), U-Net training on the device
     kernels, then the plain solver from the default image and from the
@@ -432,13 +432,16 @@ def main() -> int:
     ap.add_argument("--ws-test", type=int, default=12, help="warm start: held-out observations")
     ap.add_argument("--ws-no-graphs", action="store_true",
                     help="warm start: the solver launches every kernel (no evaluation graphs)")
-    ap.add_argument("--ws-workload", type=int, default=0, choices=(0, 4),
-                    help="warm start: observations of workload 0 (256^2, 36,000 vis) or 4 "
-                         "(512^2, 1,029,000 vis)")
+    ap.add_argument("--ws-workload", type=int, default=0, choices=(0, 1, 4),
+                    help="warm start: observations of workload 0 (256^2, 36,000 vis), 1 "
+                         "(1024^2, 1,764,000 vis) or 4 (512^2, 1,029,000 vis)")
     ap.add_argument("--ws-levels", type=int, default=None,
-                    help="warm start: U-Net levels (default 4 at 256^2, 6 at 512^2)")
+                    help="warm start: U-Net levels (default 4 at 256^2, 6 otherwise)")
     ap.add_argument("--ws-base", type=int, default=None,
-                    help="warm start: U-Net base channels (default 16 at 256^2, 32 at 512^2)")
+                    help="warm start: U-Net base channels (default 16 at 256^2 and 1024^2, 32 "
+                         "at 512^2)")
+    ap.add_argument("--ws-iterations", type=int, default=500,
+                    help="warm start: the solver's iteration cap (pairs and evaluation)")
     ap.add_argument("--ws-batch", type=int, default=8, help="warm start: training batch")
     ap.add_argument("--ws-save", metavar="FILE",
                     help="warm start: only generate the pairs of --ws-seeds and save them (.npz)")
@@ -466,18 +469,17 @@ def main() -> int:
             warmstart.GRAPHS = False
         if args.ws_save:
             a, b = (int(x) for x in (args.ws_seeds or "1000:%d" % (1000 + args.ws_pairs)).split(":"))
-            pr = generate_pairs(range(a, b), 500, args.ws_workload)
+            pr = generate_pairs(range(a, b), args.ws_iterations, args.ws_workload)
             with open(args.ws_save, "wb") as f:
                 np.savez(f, dirty=pr.dirty, model=pr.model, seeds=np.array(pr.seeds),
                          seconds=pr.seconds)
             print(json.dumps({"saved": args.ws_save, "pairs": len(pr.dirty),
                               "seconds": pr.seconds}), flush=True)
             return 0
-        big = args.ws_workload == 4
         r = bench_warm_start(n_train=args.ws_pairs, n_test=args.ws_test, steps=args.ws_steps,
-                             iterations=500,
-                             levels=args.ws_levels or (6 if big else 4),
-                             base=args.ws_base or (32 if big else 16),
+                             iterations=args.ws_iterations,
+                             levels=args.ws_levels or (4 if args.ws_workload == 0 else 6),
+                             base=args.ws_base or (32 if args.ws_workload == 4 else 16),
                              which=args.ws_workload, batch=args.ws_batch, load=args.ws_load)
         print(json.dumps(r), flush=True)
         return 0

@@ -288,9 +288,9 @@ GRAPHS = True
 
 
 def observation(seed: int, which: int = 0, **overrides):
-    """An observation of workload `which` (0: 256^2 / 36,000 visibilities; 4:
-    512^2 / 1,029,000) with the sky and the noise drawn from `seed` (the
-    array layout stays fixed)."""
+    """An observation of workload `which` (0: 256^2 / 36,000 visibilities; 1:
+    1024^2 / 1,764,000; 4: 512^2 / 1,029,000) with the sky and the noise drawn
+    from `seed` (the array layout stays fixed)."""
     from .problem import make_problem
 
     kw = dict(seed_sky=seed, seed_noise=seed + 7919)
