@@ -14,6 +14,12 @@ average duration (HIP events on the context's stream).  The cpu_baseline leg
 times the FP64 oracle on the host cores on a bounded sample (full-size
 forward chi2 + the separable FP64 gradient over a slice of the
 visibilities, extrapolated linearly in the visibility count).
+
+Other modes, each printing its own JSON line: --workload 0/2/3 (other
+workloads), --time-to-image ITERS (split precision against plain FP32 at
+equal objective, median over observations), --precision-study, --gridding,
+--warm-start (the learned warm start; --ws-* options), --graphs and
+--autotune (modifiers of the evals/s run).
 """
 from __future__ import annotations
 
