@@ -204,7 +204,8 @@ def generate_pairs(seeds, iterations: int, which: int):
 
 
 def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, levels: int,
-                     base: int, which: int = 0, batch: int = 8, load=None) -> dict:
+                     base: int, which: int = 0, batch: int = 8, load=None,
+                     eval_lbfgs: int = 0) -> dict:
     """The learned warm start on observations of workload `which` (0: 256^2 /
     36,000 visibilities; 1: 1024^2 / 1,764,000; 4: 512^2 / 1,029,000): training pairs from the plain
     solver (CG-FRPRMN to its tolerance; generated here, or loaded from the
@@ -235,7 +236,13 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
         f"# step {k} loss {loss:.3e} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
         flush=True))
     print(f"# trained in {t_train:.1f} s, loss {ws.history[-1][1]:.3e}", file=sys.stderr, flush=True)
-    r = W.evaluate(ws, range(5000, 5000 + n_test), iterations=iterations, plain=True, which=which,
+    ev = dict(iterations=iterations)
+    if eval_lbfgs:
+        # the time-to-image objective and optimiser
+        ev = dict(iterations=eval_lbfgs, optimizer="CG-LBFGS",
+                  terms=[("Chi2", -1, 0.0), ("Entropy", 0, 0.001),
+                         ("TotalSquaredVariation", 1, 0.0)], lambdas=(0.01, 1e-3))
+    r = W.evaluate(ws, range(5000, 5000 + n_test), plain=True, which=which, **ev,
                    progress=lambda row: print(
                        f"# held-out {row['seed']}: iterations {row['iters_default']} / "
                        f"{row['iters_warm']}, seconds {row['seconds_default']:.2f} / "
@@ -245,8 +252,11 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
         "metric": "learned warm start: time to the objective both starts reach, default / warm",
         "value": r["median_time_ratio"], "unit": "x (median over held-out observations)",
         "higher_is_better": True, "n_gpus": 1, "data": "synthetic (seeded generator)",
-        "config": {"workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, chi2+entropy, "
-                               f"CG-FRPRMN to its tolerance (at most {iterations} iterations)",
+        "config": {"workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, pairs from "
+                               f"chi2+entropy, CG-FRPRMN to its tolerance (at most {iterations} "
+                               "iterations); evaluated with " +
+                               (f"L-BFGS m=10 on chi2+entropy+TSV (at most {eval_lbfgs} "
+                                "iterations)" if eval_lbfgs else "the same solver"),
                    "unet": {"levels": levels, "base_channels": base, "parameters": ws.net.nparams,
                             "weights": "bf16 (fp32 master, fp32 accumulation)" if ws.net.bf16
                             else "fp32", "batch": batch},
@@ -448,6 +458,9 @@ def main() -> int:
                          "at 512^2)")
     ap.add_argument("--ws-iterations", type=int, default=500,
                     help="warm start: the solver's iteration cap (pairs and evaluation)")
+    ap.add_argument("--ws-eval-lbfgs", type=int, default=0, metavar="ITERS",
+                    help="warm start: evaluate with L-BFGS (m=10) on chi2 + entropy + TSV, the "
+                         "time-to-image objective, for at most ITERS iterations")
     ap.add_argument("--ws-batch", type=int, default=8, help="warm start: training batch")
     ap.add_argument("--ws-save", metavar="FILE",
                     help="warm start: only generate the pairs of --ws-seeds and save them (.npz)")
@@ -486,7 +499,8 @@ def main() -> int:
                              iterations=args.ws_iterations,
                              levels=args.ws_levels or (4 if args.ws_workload == 0 else 6),
                              base=args.ws_base or (32 if args.ws_workload == 4 else 16),
-                             which=args.ws_workload, batch=args.ws_batch, load=args.ws_load)
+                             which=args.ws_workload, batch=args.ws_batch, load=args.ws_load,
+                             eval_lbfgs=args.ws_eval_lbfgs)
         print(json.dumps(r), flush=True)
         return 0
     t0 = time.perf_counter()

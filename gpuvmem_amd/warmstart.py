@@ -312,14 +312,17 @@ def dirty_image(ctx: Context, p) -> np.ndarray:
 
 
 def solve(p, init: Optional[np.ndarray] = None, iterations: int = 400,
-          optimizer: str = "CG-FRPRMN", graphs: Optional[bool] = None, adjoint: str = "bf16x3"):
+          optimizer: str = "CG-FRPRMN", graphs: Optional[bool] = None, adjoint: str = "bf16x3",
+          terms=None, lambdas=None):
     """The plain MEM optimisation (CG-FRPRMN, the workload's optimiser, or
     CG-LBFGS with memory 10), run to its own tolerance or `iterations`, from
-    the default image or `init` (image 0).  Returns (final image 0,
-    trajectory of objective values with the starting value first, seconds to
-    each entry, result)."""
+    the default image or `init` (image 0), on the objective `terms` /
+    `lambdas` (default: TERMS / LAMBDAS).  Returns (final image 0, trajectory
+    of objective values with the starting value first, seconds to each entry,
+    result)."""
     ctx = Context(p, adjoint=adjoint)
-    obj = Objective(ctx, fw_config(TERMS, lambdas=LAMBDAS))
+    obj = Objective(ctx, fw_config(TERMS if terms is None else terms,
+                                   lambdas=LAMBDAS if lambdas is None else lambdas))
     obj.set_graphs(GRAPHS if graphs is None else graphs)
     img = p.initial_image()
     if init is not None:
@@ -427,7 +430,8 @@ def time_to(traj, secs, target: float):
 
 
 def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-FRPRMN",
-             plain: bool = False, which: int = 0, progress=None) -> dict:
+             plain: bool = False, which: int = 0, progress=None, terms=None,
+             lambdas=None) -> dict:
     """The plain solver from the default image and from the warm start on
     held-out observations: iterations and seconds (dirty image and inference
     included for the warm start) to the objective value both reach, and the
@@ -443,8 +447,10 @@ def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-F
         init = ws.predict(dirty[None])[0]
         t_pre = time.perf_counter() - t0
         ctx.close()
-        img_d, tr_d, sc_d, _ = solve(p, iterations=iterations, optimizer=optimizer)
-        img_w, tr_w, sc_w, _ = solve(p, init=init, iterations=iterations, optimizer=optimizer)
+        obj_kw = dict(terms=terms, lambdas=lambdas)
+        img_d, tr_d, sc_d, _ = solve(p, iterations=iterations, optimizer=optimizer, **obj_kw)
+        img_w, tr_w, sc_w, _ = solve(p, init=init, iterations=iterations, optimizer=optimizer,
+                                     **obj_kw)
         # final images against the true sky (internal units) inside the mask
         mask = p.noise < p.noise_cut
         truth = (p.sky / p.fg_scale)[mask]
@@ -455,7 +461,7 @@ def evaluate(ws: WarmStart, seeds, iterations: int = 400, optimizer: str = "CG-F
         extra = {}
         if plain:
             img_p, tr_p, sc_p, _ = solve(p, iterations=iterations, optimizer=optimizer,
-                                         graphs=False, adjoint="f32")
+                                         graphs=False, adjoint="f32", **obj_kw)
             t3 = max(target, tr_p[-1])
             cp, cw = time_to(tr_p, sc_p, t3), time_to(tr_w, sc_w, t3)
             extra = dict(f_plain=tr_p[-1], iters_plain=cp[0], seconds_plain=cp[1],
