@@ -32,9 +32,13 @@ namespace gvm {
 // ---------------------------------------------------------------------------
 // clip2IWNoise: src/functions.cu:2694-2719
 // ---------------------------------------------------------------------------
+// Block 0 also clears the per-slice max |c| that the degrid kernel raises
+// (one launch less per evaluation).
 __global__ void clip_kernel(float* __restrict__ I, const float* __restrict__ noise, long n,
                             float noise_cut, float floor_v, int eta_pos, float threshold,
-                            int flag_opt) {
+                            int flag_opt, unsigned int* __restrict__ cmax, int nsl) {
+  if (blockIdx.x == 0)
+    for (int s = threadIdx.x; s < nsl; s += blockDim.x) cmax[s] = 0u;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
        k += (long)gridDim.x * blockDim.x) {
     if (noise[k] > noise_cut) {
@@ -44,11 +48,6 @@ __global__ void clip_kernel(float* __restrict__ I, const float* __restrict__ noi
       I[n + k] = 0.0f;
     }
   }
-}
-
-__global__ void zero_u32(unsigned int* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -345,6 +344,26 @@ __global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
   }
 }
 
+// For a few slices, finish_kernel and chi2_combine_kernel in one launch:
+// one block reduces each slice's block partials with finish_kernel's thread
+// assignment and tree, then thread 0 combines them as chi2_combine_kernel
+// does, so the value is the same bit for bit.
+__global__ __launch_bounds__(kReduceThreads) void chi2_finish_kernel(
+    const double* __restrict__ partials, int nblocks, const SliceDev* __restrict__ tab, int nsl,
+    int normalize, double* __restrict__ out) {
+  __shared__ double sh[kReduceThreads / 64];
+  double total = 0.0;
+  for (int s = 0; s < nsl; s++) {
+    const double* p = partials + (long)s * kReduceBlocks;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) v = v + p[i];
+    const double r = block_sum(v, sh);
+    if (threadIdx.x == 0) total += normalize ? r / (double)tab[s].nvis : r;
+    __syncthreads();  // sh is reused by the next slice
+  }
+  if (threadIdx.x == 0) *out = 0.5 * total;
+}
+
 // 0.5 * the sum over slices of the per-slice sums (each divided by its
 // visibility count if normalize), in slice order, in FP64 (src/functions.cu:
 // 4377-4453): one thread, so the order is fixed.
@@ -368,10 +387,8 @@ gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normal
   GVM_HIP_TRY(hipSetDevice(ctx->device));
   const float floor_v = (float)(-1.0 * (double)g.eta * (double)g.minpix);
   hipLaunchKernelGGL(clip_kernel, dim3(reduce_grid(n)), dim3(256), 0, ctx->stream, I, ctx->noise,
-                     n, g.noise_cut, floor_v, g.eta > 0.0f ? 1 : 0, g.threshold, flag_opt);
-  GVM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(zero_u32, dim3((nsl + 255) / 256), dim3(256), 0, ctx->stream, ctx->cmax,
-                     nsl);
+                     n, g.noise_cut, floor_v, g.eta > 0.0f ? 1 : 0, g.threshold, flag_opt,
+                     ctx->cmax, nsl);
   GVM_CHECK_LAUNCH();
   gvm_status st = model_grids(ctx, I, nsl);
   if (st) return st;
@@ -384,11 +401,17 @@ gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normal
                      ctx->grid,
                      g.N, ctx->slice_tab, ctx->partials, ctx->cmax);
   GVM_CHECK_LAUNCH();
-  st = finish_reductions(ctx, nsl, nb);
-  if (st) return st;
-  hipLaunchKernelGGL(chi2_combine_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->dscalars,
-                     ctx->slice_tab, nsl, normalize, ctx->dvalues + slot);
-  GVM_CHECK_LAUNCH();
+  if (nsl <= 4) {
+    hipLaunchKernelGGL(chi2_finish_kernel, dim3(1), dim3(kReduceThreads), 0, ctx->stream,
+                       ctx->partials, nb, ctx->slice_tab, nsl, normalize, ctx->dvalues + slot);
+    GVM_CHECK_LAUNCH();
+  } else {
+    st = finish_reductions(ctx, nsl, nb);
+    if (st) return st;
+    hipLaunchKernelGGL(chi2_combine_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->dscalars,
+                       ctx->slice_tab, nsl, normalize, ctx->dvalues + slot);
+    GVM_CHECK_LAUNCH();
+  }
   ctx->have_residuals = true;
   return GVM_OK;
 }
