@@ -64,8 +64,8 @@ def cpu_baseline(p, threads: int, target_s: float) -> dict:
     _, vr, clipped = orc.chi2(img)
     t_fwd = time.perf_counter() - t0
     # gradient on a slice of the visibilities (of the first channel); grow the
-    # slice until it costs about target_s seconds, then extrapolate linearly
-    # in the total visibility count
+    # slice until it costs about half of target_s (the growing rounds take the
+    # rest), then extrapolate linearly in the total visibility count
     n = p.nvis_total
     frac_n = max(256, p.nvis // 512)
     while True:
@@ -73,9 +73,9 @@ def cpu_baseline(p, threads: int, target_s: float) -> dict:
         t0 = time.perf_counter()
         sub.dchi2_channel(vr[:frac_n])
         dt = time.perf_counter() - t0
-        if dt > target_s / 4 or frac_n >= p.nvis:
+        if dt > target_s / 2 or frac_n >= p.nvis:
             break
-        frac_n = min(p.nvis, int(frac_n * max(2.0, (target_s / 4) / max(dt, 1e-3))))
+        frac_n = min(p.nvis, int(frac_n * max(2.0, (target_s / 2) / max(dt, 1e-3))))
     t_grad = dt * n / frac_n
     evals = 1.0 / (t_fwd + t_grad)
     grid_s, grid_k = gridding_comparator(p, oracle, threads)
