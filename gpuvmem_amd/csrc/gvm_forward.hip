@@ -195,7 +195,14 @@ __global__ __launch_bounds__(T) void fft_cols_kernel(float2* __restrict__ grids,
                                                        const float2* __restrict__ rot_y) {
   __shared__ float2 lds[B * N];
   float2* __restrict__ grid = grids + (long)blockIdx.y * N * N;
-  const int col0 = blockIdx.x * B;
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs
+  // (blockIdx.x % 8), each with its own L2; the blocks whose column groups
+  // share 128-byte lines are given consecutive groups on one XCD, so a line
+  // is fetched into one L2 instead of up to eight.
+  const int nbx = gridDim.x;
+  const int gx = (nbx % 8 == 0) ? (int)(blockIdx.x % 8) * (nbx / 8) + (int)(blockIdx.x / 8)
+                                : (int)blockIdx.x;
+  const int col0 = gx * B;
   for (int idx = threadIdx.x; idx < B * N; idx += T) {
     const int p = idx / B, t = idx % B;
     lds[idx] = grid[(long)p * N + col0 + t];
