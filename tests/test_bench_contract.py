@@ -1,0 +1,41 @@
+"""The parts of bench.py that run without a GPU: the HBM traffic figures it
+takes from the committed rocprofv3 counter summaries, and the refusal of more
+than one GPU."""
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+import bench  # noqa: E402
+
+NVIS = 1_764_000
+# the GEMM writes one FP32 1024 x 1024 partial image per visibility range: ranges of
+# 3072 visibilities, their count padded to a multiple of 8 (575 -> 576)
+RANGES = -(-(-(-NVIS // 3072)) // 8) * 8
+PARTIAL_BYTES = RANGES * 1024 * 1024 * 4
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "f16x3"])
+def test_traffic_summary_for_each_mfma_form(mode):
+    traffic, source = bench.profiled_traffic(mode, NVIS)
+    assert source is not None, f"no counter summary for {mode} at {NVIS} visibilities"
+    assert (ROOT / "profiles" / source).exists()
+    # at least the partial tiles it writes, at most a few percent more than that plus reads
+    assert PARTIAL_BYTES <= traffic <= 1.05 * PARTIAL_BYTES
+    text = (ROOT / "profiles" / source).read_text()
+    assert f"--adjoint {mode}" in text or mode == "bf16x3"
+
+
+def test_no_summary_for_an_unprofiled_size():
+    assert bench.profiled_traffic("bf16x3", NVIS + 1) == (None, None)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
