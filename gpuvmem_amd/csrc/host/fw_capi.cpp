@@ -20,6 +20,7 @@ struct gvm_fw_objective {
   std::unique_ptr<ObjectiveFunction> of;
   imageMap* mapping = nullptr;
   bool lbfgs_host_scalars = false;
+  std::vector<double> replay;  // line-search step lengths to replay (empty: search)
   ~gvm_fw_objective() { delete[] mapping; }
 };
 
@@ -148,6 +149,10 @@ extern "C" gvm_status gvm_fw_optimize_timed(gvm_fw_objective* obj, const char* o
     opt->setFlag(obj->cfg.flag_opt);
     const long f0 = obj->of->getFunctionCalls(), g0 = obj->of->getGradientCalls();
     const auto t0 = std::chrono::steady_clock::now();
+    setLineSearchReplay(obj->replay.data(), (int)obj->replay.size());
+    struct ReplayReset {
+      ~ReplayReset() { setLineSearchReplay(nullptr, 0); }
+    } replay_reset;
     opt->optimize();
     gvm::check(gvm_sync(obj->ctx), "gvm_sync");
     const auto t1 = std::chrono::steady_clock::now();
@@ -168,6 +173,16 @@ extern "C" gvm_status gvm_fw_optimize_timed(gvm_fw_objective* obj, const char* o
   } catch (const std::exception& e) {
     return fail(e);
   }
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_set_line_search_replay(gvm_fw_objective* obj, const double* xmins,
+                                                    int n) {
+  if (!obj || n < 0 || (n > 0 && !xmins)) {
+    g_fw_error = "gvm_fw_set_line_search_replay: bad argument";
+    return GVM_ERR_ARG;
+  }
+  obj->replay.assign(xmins, xmins + n);
   return GVM_OK;
 }
 

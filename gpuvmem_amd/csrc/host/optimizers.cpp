@@ -222,14 +222,36 @@ double f1dim(double x) {
   return testof->calcFunctionD(device_xt);
 }
 
+// Line-search replay (not in the reference): when a table of step lengths
+// is set, linmin takes the next one as xmin instead of bracketing and
+// minimising, and returns f1dim at that step (what brent returns at its
+// minimum).  Replaying another implementation's accepted steps separates
+// arithmetic differences from the line search's branch decisions.
+namespace {
+std::vector<double> g_replay;
+size_t g_replay_pos = 0;
+}  // namespace
+
+void setLineSearchReplay(const double* xmins, int n) {
+  g_replay.assign(xmins, xmins + std::max(0, n));
+  g_replay_pos = 0;
+}
+
 void linmin(float* p, float* xi, double* fret, f1dim_fn func) {
   const long tot = img_n() * globals().image_count;
   (void)func;
   check(gvm_vec_copy(ctx(), device_pcom, p, tot), "gvm_vec_copy(pcom)");
   check(gvm_vec_copy(ctx(), device_xicom, xi, tot), "gvm_vec_copy(xicom)");
   double ax = 0.0, xx = 1.0, bx, fa, fx, fb, xmin = 0.0;
-  mnbrak(&ax, &xx, &bx, &fa, &fx, &fb, f1dim);
-  *fret = brent(ax, xx, bx, 1.0e-7, &xmin, f1dim);
+  if (!g_replay.empty()) {
+    if (g_replay_pos >= g_replay.size())
+      throw gvm::Error("linmin: the line-search replay table is exhausted", GVM_ERR_STATE);
+    xmin = g_replay[g_replay_pos++];
+    *fret = f1dim(xmin);
+  } else {
+    mnbrak(&ax, &xx, &bx, &fa, &fx, &fb, f1dim);
+    *fret = brent(ax, xx, bx, 1.0e-7, &xmin, f1dim);
+  }
   if (globals().verbose) std::printf("Alpha for linear minimization = %f\n\n", xmin);
   for (int i = 0; i < I->getImageCount(); i++) (I->getFunctionMapping()[i].newP)(p, xi, xmin, i);
 }

@@ -320,6 +320,14 @@ class Objective:
         native.check_fw(self.fw.gvm_fw_set_lbfgs_host_scalars(self.h, int(on)),
                         "gvm_fw_set_lbfgs_host_scalars")
 
+    def set_line_search_replay(self, xmins) -> None:
+        """The next optimize() takes its line-search step lengths from xmins
+        in turn instead of searching (an empty list restores the search)."""
+        arr = np.ascontiguousarray(np.asarray(xmins, np.float64))
+        ptr = arr.ctypes.data_as(C.POINTER(C.c_double)) if arr.size else None
+        native.check_fw(self.fw.gvm_fw_set_line_search_replay(self.h, ptr, int(arr.size)),
+                        "gvm_fw_set_line_search_replay")
+
     def graph_launches(self) -> int:
         n = C.c_long()
         native.check_fw(self.fw.gvm_fw_graph_launches(self.h, C.byref(n)), "gvm_fw_graph_launches")

@@ -89,6 +89,13 @@ gvm_status gvm_fw_graph_launches(gvm_fw_objective* obj, long* n);
  * scalars stay on the device and the recursion runs without host round
  * trips.  Both give the same search direction, bit for bit. */
 gvm_status gvm_fw_set_lbfgs_host_scalars(gvm_fw_objective* obj, int on);
+/* Line-search replay (no counterpart in the reference): the next
+ * gvm_fw_optimize takes xmins[0..n) in turn as linmin's step lengths instead of
+ * bracketing and minimising (src/linmin.cu:52-110), and fails with
+ * GVM_ERR_STATE if it needs more.  n = 0 restores the search.  Replaying the
+ * steps another implementation accepted (the FP64 oracle's) compares two
+ * optimiser paths without the line search's branch decisions. */
+gvm_status gvm_fw_set_line_search_replay(gvm_fw_objective* obj, const double* xmins, int n);
 const char* gvm_fw_last_error(void);
 
 /* 1 if `name` is registered with the factory of `kind` ("Fi", "Optimizer",

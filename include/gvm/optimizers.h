@@ -50,3 +50,6 @@ void mnbrak(double* ax, double* bx, double* cx, double* fa, double* fb, double* 
 double brent(double ax, double bx, double cx, double tol, double* xmin, f1dim_fn f);
 double f1dim(double x);
 void linmin(float* p, float* xi, double* fret, f1dim_fn func);
+// Not in the reference: linmin takes its step lengths from xmins[0..n) in
+// turn instead of searching (n = 0 restores the search).
+void setLineSearchReplay(const double* xmins, int n);

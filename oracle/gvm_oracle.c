@@ -905,6 +905,19 @@ static double f1dim(double x, void* vctx) {
  * p <- p + xi with the image-0 floor (xi set to 0 where floored). */
 static long g_linmin_calls = 0;
 
+/* Optional log of the accepted step lengths (xmin of each linmin), for
+ * replaying this line search's decisions in another implementation. */
+static double* g_xmin_log = NULL;
+static int g_xmin_cap = 0, g_xmin_count = 0;
+
+void orc_set_xmin_log(double* buf, int cap) {
+  g_xmin_log = buf;
+  g_xmin_cap = buf ? cap : 0;
+  g_xmin_count = 0;
+}
+
+int orc_xmin_log_count(void) { return g_xmin_count; }
+
 double orc_linmin(const orc_problem* pr, double* p, double* xi) {
   long n = pr->n;
   long tot = n * pr->image_count;
@@ -918,6 +931,8 @@ double orc_linmin(const orc_problem* pr, double* p, double* xi) {
   orc_mnbrak(&ax, &xx, &bx, &fa, &fx, &fb, f1dim, &c);
   double fret = orc_brent(ax, xx, bx, 1.0e-7, &xmin, f1dim, &c);
   g_linmin_calls += c.calls;
+  if (g_xmin_log && g_xmin_count < g_xmin_cap) g_xmin_log[g_xmin_count] = xmin;
+  if (g_xmin_log) g_xmin_count++;
   double floor_v = pr->floor_v;
   for (int im = 0; im < pr->image_count; im++) {
     double* pp = p + n * im;

@@ -191,6 +191,10 @@ void orc_mnbrak(double* ax, double* bx, double* cx, double* fa, double* fb,
                 double* fc, orc_f1 f, void* ctx);
 double orc_brent(double ax, double bx, double cx, double tol, double* xmin,
                  orc_f1 f, void* ctx);
+/* Log of linmin's accepted step lengths: buf[0..cap) receives the xmin of
+ * each later linmin call (NULL stops logging); count is the number of calls. */
+void orc_set_xmin_log(double* buf, int cap);
+int orc_xmin_log_count(void);
 double orc_linmin(const orc_problem* pr, double* p, double* xi);
 
 typedef struct {

@@ -128,6 +128,8 @@ def lib() -> C.CDLL:
             "orc_brent": ([D, D, D, D, P, F1, P], D),
             "orc_of_problem": ([C.POINTER(OObjective)], OProblem),
             "orc_linmin": ([C.POINTER(OProblem), P, P], D),
+            "orc_set_xmin_log": ([P, I], None),
+            "orc_xmin_log_count": ([], I),
             "orc_frprmn": ([C.POINTER(OProblem), P, I, D, D, C.POINTER(OResult), P, I], I),
             "orc_lbfgs": ([C.POINTER(OProblem), P, I, I, D, D, C.POINTER(OResult), P, I], I),
             "orc_do_gridding_pillbox": ([I, I, D, D, CP, I, I, I, P, P, P], None),
@@ -373,11 +375,23 @@ class Oracle:
         return xi
 
     def optimize(self, of, image, optimizer: str = "CG-FRPRMN", iterations: int = 50,
-                 ftol: float = 1e-12, gtol: float = 1e-12, lbfgs_k: int = 100):
-        """Returns (final image, OResult, values after each line search)."""
+                 ftol: float = 1e-12, gtol: float = 1e-12, lbfgs_k: int = 100,
+                 steps: bool = False):
+        """Returns (final image, OResult, values after each line search), and
+        with steps=True also the step length linmin accepted at each line
+        search."""
         img = np.ascontiguousarray(image, np.float64).copy()
         pr = self._lib.orc_of_problem(C.byref(of))
-        return run_optimizer(pr, img, optimizer, iterations, ftol, gtol, lbfgs_k)
+        if not steps:
+            return run_optimizer(pr, img, optimizer, iterations, ftol, gtol, lbfgs_k)
+        log = np.zeros(max(1, iterations + 1), np.float64)
+        self._lib.orc_set_xmin_log(_ptr(log), len(log))
+        try:
+            out = run_optimizer(pr, img, optimizer, iterations, ftol, gtol, lbfgs_k)
+            count = int(self._lib.orc_xmin_log_count())
+        finally:
+            self._lib.orc_set_xmin_log(None, 0)
+        return (*out, log[: min(count, len(log))].copy())
 
 
     def optimize_schedule(self, of, image, flags=(0, 1, 2, 3), **kw):

@@ -109,6 +109,64 @@ def test_cg_frprmn_trajectory_workload0():
     assert ierr < 0.1
 
 
+def _replay(p, optimizer, iterations, terms, lambdas, k=0, mode="bf16x3"):
+    """The FP64 oracle's run, then the device run with the oracle's accepted
+    line-search steps replayed: both take the same step lengths, so what
+    remains between the two final images is the arithmetic (FP32 device
+    kernels, split-precision adjoint) carried through every search direction
+    (Polak-Ribiere gamma, L-BFGS recursion), the image updates and their
+    positivity floor."""
+    orc = oracle.Oracle(p)
+    of = orc.objective(terms, lambdas=lambdas)
+    ref, ores, otraj, xmins = orc.optimize(of, p.initial_image(), optimizer, iterations,
+                                           lbfgs_k=k or 100, steps=True)
+    orc.free(of)
+    assert len(xmins) == ores.iterations
+    ctx = Context(p, adjoint=mode)
+    obj = Objective(ctx, fw_config(terms, lambdas=lambdas))
+    obj.set_line_search_replay(xmins)
+    d = ctx.image(p.initial_image())
+    res, traj = obj.optimize(d, optimizer, ores.iterations, lbfgs_k=k)
+    img = d.download(np.float32, (2, p.N, p.M))
+    obj.close()
+    ctx.close()
+    ierr = np.linalg.norm(img - ref) / np.linalg.norm(ref)
+    n = min(len(traj), len(otraj))
+    terr = np.abs(np.array(traj[:n]) - otraj[:n]) / np.abs(otraj[:n])
+    print(f"{optimizer}: {ores.iterations} replayed steps; final image rel L2 {ierr:.3e}; "
+          f"values max rel err {terr.max():.3e}; final f {res.final_value:.10e} vs "
+          f"{ores.final_value:.10e}")
+    return ierr, terr, res, ores
+
+
+def test_cg_frprmn_final_image_with_replayed_steps_workload0():
+    """CG-FRPRMN, chi2 + entropy, 50 iterations on the 256^2 / 36,000-vis
+    workload (the configuration's optimiser and count), with the oracle's
+    step lengths replayed: the final image is within 1e-4 relative L2 of the
+    FP64 oracle's, and every line-search value within 1e-5."""
+    p = make_problem(0)
+    print(f"case: CG-FRPRMN replay 50 it, {p.M}^2, nvis={p.nvis}", flush=True)
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001)]
+    ierr, terr, res, ores = _replay(p, "CG-FRPRMN", 50, terms, (0.01,))
+    assert ores.iterations == 50 and res.iterations == 50
+    assert ierr < 1e-4
+    assert np.all(terr < 1e-5)
+
+
+def test_lbfgs_final_image_with_replayed_steps_workload0():
+    """L-BFGS memory 10, chi2 + entropy + TSV, on the 256^2 workload until
+    the oracle's own exit (its line search stops making progress after about
+    20 iterations), with the oracle's step lengths replayed: the final image
+    is within 1e-4 relative L2 of the FP64 oracle's."""
+    p = make_problem(0)
+    print(f"case: CG-LBFGS replay, {p.M}^2, nvis={p.nvis}", flush=True)
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
+    ierr, terr, res, ores = _replay(p, "CG-LBFGS", 50, terms, (0.01, 1e-3), k=10)
+    assert res.iterations == ores.iterations
+    assert ierr < 1e-4
+    assert np.all(terr < 1e-5)
+
+
 def test_trajectory_is_bitwise_deterministic():
     """Fixed-order reductions: two runs give identical trajectories and images."""
     p = small_problem(npix=128, n_antennas=8, seed=31)

@@ -378,6 +378,112 @@ def test_lbfgs_reference_indexing_is_memory_one_for_image0():
     assert t[-1] < t[0]
 
 
+def _exact_line_min(f, x, d):
+    from scipy.optimize import minimize_scalar
+
+    a = minimize_scalar(lambda t: f(x + t * d), bracket=(0.0, 1.0), tol=1e-14).x
+    return x + a * d
+
+
+def test_cg_gamma_is_polak_ribiere_plus_by_hand():
+    """src/frprmn.cu:154-175: gg = sum g^2, dgg = sum (xi + g) xi with xi the
+    new gradient and g the old negated gradient, gam = max(0, dgg / gg),
+    h <- -xi + gam h.  The path is recomputed here with exact line searches
+    on a convex non-quadratic function.  At step 2 Polak-Ribiere and
+    Fletcher-Reeves coincide (exact line search makes the gradients
+    orthogonal); at step 3 they do not, and the oracle follows PR."""
+    r = np.random.default_rng(7)
+    n = 3
+    Q = np.linalg.qr(r.standard_normal((n, n)))[0]
+    A = Q @ np.diag([1.0, 3.0, 7.0]) @ Q.T
+    b = r.standard_normal(n)
+    wq = np.array([0.5, 1.0, 2.0])
+    f = lambda p: 0.5 * p @ A @ p - b @ p + np.sum(wq * p ** 4)
+    grad = lambda p: A @ p - b + 4.0 * wq * p ** 3
+
+    def by_hand(x0, iters, rule):
+        x, gk = x0.copy(), grad(x0)
+        d = -gk
+        pts = []
+        for _ in range(iters):
+            x = _exact_line_min(f, x, d)
+            pts.append(x.copy())
+            gn = grad(x)
+            if rule == "PR+":
+                gam = max(0.0, float((gn - gk) @ gn) / float(gk @ gk))
+            else:  # Fletcher-Reeves
+                gam = float(gn @ gn) / float(gk @ gk)
+            d = -gn + gam * d
+            gk = gn
+        return pts
+
+    x0 = np.array([1.0, -1.0, 0.5])
+    pr, fr = by_hand(x0, 3, "PR+"), by_hand(x0, 3, "FR")
+    pp = oracle.PyProblem(n, 1, f, lambda p, it: grad(p))
+    for it in (1, 2, 3):
+        x, res, _ = pp.optimize(x0, "CG-FRPRMN", iterations=it)
+        assert res.iterations == it
+        assert np.linalg.norm(x - pr[it - 1]) < 1e-6, it
+    # the check tells the two rules apart: FR's third point is far away
+    assert np.linalg.norm(pr[2] - fr[2]) > 1e-2
+    x3, _, _ = pp.optimize(x0, "CG-FRPRMN", iterations=3)
+    assert np.linalg.norm(x3 - fr[2]) > 1e-2
+
+
+def test_lbfgs_y_and_h0_by_hand():
+    """src/lbfgs.cu:139-180 with calculateSandY (src/functions.cu:3635-3653):
+    y = xi_new - (-xi_old), where xi_old is the search direction copied
+    before linmin, so y = g_new + d_old (= g_new - g_old only on the first
+    iteration); s = p - p_old; the initial Hessian scale is
+    sum_images s.y / y.y (src/lbfgs.cu:270-289).  With one image the
+    reference's storage addressing leaves a memory-1 two-loop recursion.
+    Recomputed here with exact line searches on a 4-D quadratic: the oracle
+    follows it, and the textbook y = g_new - g_old, or an unscaled H0, ends
+    measurably elsewhere after three iterations."""
+    r = np.random.default_rng(11)
+    n = 4
+    Q = np.linalg.qr(r.standard_normal((n, n)))[0]
+    A = Q @ np.diag([1.0, 2.5, 6.0, 15.0]) @ Q.T
+    b = r.standard_normal(n)
+    f = lambda p: 0.5 * p @ A @ p - b @ p
+    grad = lambda p: A @ p - b
+
+    def by_hand(x0, iters, y_rule="reference", scale_h0=True):
+        x, gk = x0.copy(), grad(x0)
+        d = -gk
+        pts = []
+        for _ in range(iters):
+            a = -float(grad(x) @ d) / float(d @ A @ d)
+            xn = x + a * d
+            gn = grad(xn)
+            s = xn - x
+            y = gn + d if y_rule == "reference" else gn - gk
+            rho = 1.0 / float(y @ s)
+            alpha = rho * float(s @ gn)
+            q = gn - alpha * y
+            h0 = float(s @ y) / float(y @ y) if scale_h0 else 1.0
+            rr = q * h0
+            beta = rho * float(y @ rr)
+            rr = rr + (alpha - beta) * s
+            d = -rr
+            x, gk = xn, gn
+            pts.append(x.copy())
+        return pts
+
+    x0 = np.zeros(n)
+    ref = by_hand(x0, 3)
+    textbook = by_hand(x0, 3, y_rule="textbook")
+    unscaled = by_hand(x0, 3, scale_h0=False)
+    pp = oracle.PyProblem(n, 1, f, lambda p, it: grad(p))
+    for it in (1, 2, 3):
+        x, res, _ = pp.optimize(x0, "CG-LBFGS", iterations=it, lbfgs_k=10)
+        assert res.iterations == it
+        assert np.linalg.norm(x - ref[it - 1]) < 1e-6, it
+    x3, _, _ = pp.optimize(x0, "CG-LBFGS", iterations=3, lbfgs_k=10)
+    assert np.linalg.norm(x3 - textbook[2]) > 1e-2
+    assert np.linalg.norm(x3 - unscaled[2]) > 1e-2
+
+
 def test_objective_gating_and_chi2_first(prob):
     """ObjectiveFunction: priors are 0 at iteration 0 and active after a
     gradient call with iter > 0; Chi2 overwrites dphi (it must be first)."""
