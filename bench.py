@@ -53,10 +53,48 @@ def bench_image(p) -> np.ndarray:
     return img
 
 
-def cpu_baseline(p, threads: int, target_s: float) -> dict:
+def host_cpus() -> dict:
+    """The host's processor count (nproc: the CPUs this process may run on),
+    os.cpu_count(), the CPU quota of the process' cgroup (cpu.max, if any),
+    the processor model (lscpu) and the thread count the CPU legs use: nproc,
+    capped by the quota when there is one."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    threads = nproc if quota is None else max(1, min(nproc, int(quota)))
+    return {"nproc": nproc, "os_cpu_count": os.cpu_count(), "cgroup_cpu_quota": quota,
+            "cpu_model": model, "threads": threads}
+
+
+def cpu_baseline(p, hc: dict, target_s: float) -> dict:
+    """The FP64 oracle on the host cores: full-size forward chi2, the
+    separable FP64 gradient on a slice of the visibilities (extrapolated
+    linearly in the visibility count) and the reference's direct-sum
+    gradient (src/functions.cu:3698-3791) on a sample of masked pixels
+    (extrapolated linearly in the masked pixel count).  `value` uses the
+    faster gradient form."""
+    threads = hc["threads"]
     sys.path.insert(0, str(ROOT / "oracle"))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import oracle
+
+    oracle.set_threads(threads)
 
     img = bench_image(p).astype(np.float64)
     orc = oracle.Oracle(p, separable=True)
@@ -77,16 +115,41 @@ def cpu_baseline(p, threads: int, target_s: float) -> dict:
             break
         frac_n = min(p.nvis, int(frac_n * max(2.0, (target_s / 2) / max(dt, 1e-3))))
     t_grad = dt * n / frac_n
-    evals = 1.0 / (t_fwd + t_grad)
+    # direct sums (one sincos per pixel and visibility) on masked pixels, a
+    # multiple of the thread count, grown to about a quarter of target_s
+    inside = np.argwhere(p.noise < p.noise_cut)
+    rng = np.random.default_rng(3)
+    npx = threads
+    while True:
+        pix = inside[rng.choice(len(inside), min(npx, len(inside)), replace=False)]
+        t0 = time.perf_counter()
+        orc.dchi2_pixels(vr, pix)
+        dtd = time.perf_counter() - t0
+        if dtd > target_s / 4 or npx >= len(inside):
+            break
+        npx = min(len(inside), threads * max(2 * (npx // threads),
+                                             int(npx / threads * (target_s / 4) / max(dtd, 1e-3))))
+    npx = len(pix)
+    t_direct = dtd * len(inside) / npx
+    evals = 1.0 / (t_fwd + min(t_grad, t_direct))
     grid_s, grid_k = gridding_comparator(p, oracle, threads)
     return {
         "value": evals,
         "unit": "evals/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"FP64 oracle: full-size forward chi2 ({t_fwd:.2f} s) + separable FP64 "
-                   f"gradient over {frac_n} of {n} visibilities ({dt:.2f} s), gradient "
-                   f"extrapolated x{n / frac_n:.1f} to {t_grad:.1f} s"),
+        **{k: hc[k] for k in ("nproc", "os_cpu_count", "cgroup_cpu_quota", "cpu_model")},
+        "sample": (f"FP64 oracle on {threads} threads: full-size forward chi2 ({t_fwd:.2f} s) + "
+                   f"separable FP64 gradient over {frac_n} of {n} visibilities ({dt:.2f} s), "
+                   f"extrapolated x{n / frac_n:.1f} to {t_grad:.1f} s; value uses the faster of "
+                   f"that and the direct sum"),
+        "gradient_seconds": {"separable_fp64": t_grad, "direct_sum_fp64": t_direct},
+        "direct_sum": {
+            "evals_per_s": 1.0 / (t_fwd + t_direct),
+            "sample": (f"direct FP64 sums (the reference's DChi2 algorithm) over all {n} "
+                       f"visibilities at {npx} of {len(inside)} masked pixels ({dtd:.2f} s), "
+                       f"extrapolated x{len(inside) / npx:.0f} to {t_direct:.0f} s"),
+        },
         "gridding_comparator": {
             "seconds_per_grid": grid_s,
             "threads": threads,
@@ -272,15 +335,18 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
     return out
 
 
-def profiled_traffic(mode: str, nvis: int):
+def profiled_traffic(mode: str, nvis: int, split_vis: int):
     """HBM bytes per launch of the adjoint GEMM from the committed rocprofv3
     counter summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
-    known size), when one matches this adjoint form and visibility count."""
-    pat = re.compile(r"^traffic_bytes_per_launch mode=(\S+) nvis=(\d+) bytes=(\d+)", re.M)
+    known size), when one matches this adjoint form, visibility count and
+    range length (the number of partial images the GEMM writes follows from
+    the last two)."""
+    pat = re.compile(r"^traffic_bytes_per_launch mode=(\S+) nvis=(\d+) split_vis=(\d+) "
+                     r"bytes=(\d+)", re.M)
     for f in sorted((ROOT / "profiles").glob("*.md")):
         for m in pat.finditer(f.read_text()):
-            if m.group(1) == mode and int(m.group(2)) == nvis:
-                return int(m.group(3)), f.name
+            if m.group(1) == mode and int(m.group(2)) == nvis and int(m.group(3)) == split_vis:
+                return int(m.group(4)), f.name
     return None, None
 
 
@@ -320,15 +386,16 @@ def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
     earlier when its line search makes no progress).  The target is the
     objective value every form reached (the largest of the final values),
     and each form is timed to its first iteration at or below it.  Each
-    context does one untimed evaluation first (workspace allocation).  The
-    plain FP32 form runs as it is; the split forms also replay evaluation
-    graphs and use the autotuned adjoint range length."""
+    context does one untimed evaluation first (workspace allocation).  Every
+    form runs with the same settings: evaluation graphs replayed and the
+    adjoint range length autotuned, so the ratio measures the adjoint's
+    precision form alone."""
     from gpuvmem_amd.device import Context, Objective, fw_config
 
     terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
     out, runs, images = {}, {}, {}
     for mode in forms:
-        extras = mode != "f32"
+        extras = mode != "direct"
         ctx = Context(p, adjoint=mode)
         obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
         obj.set_graphs(extras)
@@ -371,6 +438,45 @@ def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
         out[mode]["image_rel_l2_vs_" + forms[0]] = float(
             np.linalg.norm(images[mode] - ref) / np.linalg.norm(ref))
     return {"target_objective": target, "forms": out}
+
+
+def tti_observations(p, workload: int, iterations: int, n_obs: int) -> dict:
+    """time_to_image on n_obs observations (new sky and noise seeds, same
+    array): the line search is chaotic, so where each form stops, and the
+    ratio at the common objective, vary from one observation to the next.
+    Returns the median ratio of plain FP32 seconds to split-precision
+    seconds with the per-observation records."""
+    obs = []
+    for k in range(max(1, n_obs)):
+        pk = p if k == 0 else make_problem_of(workload, seed_sky=p.sim.seed_sky + 1000 * k,
+                                              seed_noise=p.sim.seed_noise + 1000 * k)
+        r = time_to_image(pk, iterations)
+        f32, split = r["forms"]["f32"], r["forms"]["bf16x3"]
+        r["ratio"] = (f32["seconds_to_target"] / split["seconds_to_target"]
+                      if split["seconds_to_target"] else None)
+        r["per_iteration_ratio"] = (f32["seconds_per_iteration"] /
+                                    split["seconds_per_iteration"])
+        r["seed_sky"] = int(pk.sim.seed_sky)
+        print(f"# observation {k}: ratio {r['ratio']:.3f}, per iteration "
+              f"{r['per_iteration_ratio']:.3f}", file=sys.stderr, flush=True)
+        obs.append(r)
+    ratios = [o["ratio"] for o in obs if o["ratio"]]
+    return {
+        "value": float(np.median(ratios)) if ratios else None,
+        "unit": "x (plain FP32 seconds / split-precision seconds, median over observations)",
+        "config": {"workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, chi2+entropy+TSV, "
+                               f"L-BFGS m=10, at most {iterations} iterations per form",
+                   "observations": len(obs),
+                   "settings": "both forms: evaluation graphs, autotuned adjoint range length"},
+        "median_per_iteration_ratio": float(np.median([o["per_iteration_ratio"] for o in obs])),
+        "observations": obs,
+    }
+
+
+def make_problem_of(workload: int, **overrides):
+    from gpuvmem_amd.problem import make_problem
+
+    return make_problem(workload, **overrides)
 
 
 def precision_study(p, pixels: int = 64) -> dict:
@@ -429,8 +535,12 @@ def main() -> int:
                          "multi-frequency 512^2 I_nu_0 + alpha, 4 spw x 64 channels, ~61M vis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--tti-observations", type=int, default=5,
+    ap.add_argument("--tti-observations", type=int, default=3,
                     help="time-to-image: observations (sky and noise seeds) to take the median over")
+    ap.add_argument("--tti-iterations", type=int, default=100,
+                    help="time-to-image leg of the default run: L-BFGS iteration cap per form")
+    ap.add_argument("--no-tti", action="store_true",
+                    help="default run: leave out the time-to-image and plain-FP32 legs")
     ap.add_argument("--time-to-image", type=int, default=0, metavar="ITERS",
                     help="instead of evals/s: L-BFGS time-to-image of the split-precision "
                          "adjoint against the plain FP32 one at equal objective value")
@@ -514,41 +624,17 @@ def main() -> int:
                           "n_gpus": 1, "data": "synthetic", **r}), flush=True)
         return 0
     if args.gridding:
-        threads = 0 if args.no_cpu_baseline else min(16, os.cpu_count() or 1)
+        threads = 0 if args.no_cpu_baseline else host_cpus()["threads"]
         r = bench_gridding(p, args.steps, args.warmup, threads, args.adjoint)
         print(json.dumps({"metric": f"gridded-visibility set-up and step ({p.M}^2, "
                                     f"{p.nvis_total} vis)", "n_gpus": 1, "data": "synthetic",
                           **r}), flush=True)
         return 0
     if args.time_to_image > 0:
-        # several observations (new sky and noise seeds, same array): the line
-        # search is chaotic, so where each form stops, and the ratio at the
-        # common objective, vary from one observation to the next
-        obs = []
-        for k in range(max(1, args.tti_observations)):
-            pk = p if k == 0 else make_problem(args.workload, seed_sky=p.sim.seed_sky + 1000 * k,
-                                               seed_noise=p.sim.seed_noise + 1000 * k)
-            r = time_to_image(pk, args.time_to_image)
-            f32, split = r["forms"]["f32"], r["forms"]["bf16x3"]
-            r["ratio"] = (f32["seconds_to_target"] / split["seconds_to_target"]
-                          if split["seconds_to_target"] else None)
-            r["per_iteration_ratio"] = (f32["seconds_per_iteration"] /
-                                        split["seconds_per_iteration"])
-            r["seed_sky"] = int(pk.sim.seed_sky)
-            print(f"# observation {k}: ratio {r['ratio']:.3f}, per iteration "
-                  f"{r['per_iteration_ratio']:.3f}", file=sys.stderr, flush=True)
-            obs.append(r)
-        ratios = [o["ratio"] for o in obs if o["ratio"]]
+        r = tti_observations(p, args.workload, args.time_to_image, args.tti_observations)
         print(json.dumps({
             "metric": "time-to-image vs plain FP32 at equal objective (1024^2, 1.76M vis)",
-            "value": float(np.median(ratios)) if ratios else None,
-            "unit": "x (plain FP32 seconds / split-precision seconds, median over observations)",
-            "higher_is_better": True, "n_gpus": 1, "data": "synthetic",
-            "config": {"workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, chi2+entropy+TSV, "
-                                   f"L-BFGS m=10, {args.time_to_image} iterations of the FP32 form",
-                       "observations": len(obs)},
-            "median_per_iteration_ratio": float(np.median([o["per_iteration_ratio"] for o in obs])),
-            "observations": obs}), flush=True)
+            "higher_is_better": True, "n_gpus": 1, "data": "synthetic", **r}), flush=True)
         return 0
     ctx = Context(p, adjoint=args.adjoint)
     terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
@@ -591,7 +677,7 @@ def main() -> int:
 
     nvt = p.nvis_total
     nsl = len(p.all_slices())
-    traffic, traffic_src = profiled_traffic(args.adjoint, nvt)
+    traffic, traffic_src = profiled_traffic(args.adjoint, nvt, ctx.split_vis())
     flops = 4.0 * p.M * p.N * nvt
     achieved = flops / (adj_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.adjoint]
@@ -636,15 +722,66 @@ def main() -> int:
             "algorithmic_flop_per_launch": flops,
         },
     }
-    if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(p, threads, args.cpu_seconds)
-    print(json.dumps(line), flush=True)
     if args.calibrate_copy_mib > 0:
         copy_calibration(ctx, args.calibrate_copy_mib)
     obj.close()
     ctx.close()
+    if not args.no_tti and args.workload == 1 and args.adjoint != "f32":
+        # the plain FP32 form's evals/s on the same step, then time to image
+        line["plain_fp32"] = plain_fp32_step(p, args.steps, args.warmup, args.graphs)
+        print(f"# plain FP32 step: {line['plain_fp32']}", file=sys.stderr, flush=True)
+        tti = tti_observations(p, args.workload, args.tti_iterations, args.tti_observations)
+        line["time_to_image"] = compact_tti(tti)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(p, host_cpus(), args.cpu_seconds)
+    print(json.dumps(line), flush=True)
     return 0
+
+
+def plain_fp32_step(p, steps: int, warmup: int, graphs: bool) -> dict:
+    """The benchmark's step with the plain FP32 MFMA adjoint."""
+    from gpuvmem_amd.device import Context, Objective, fw_config
+
+    ctx = Context(p, adjoint="f32")
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
+    obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
+    obj.set_graphs(graphs)
+    d_img = ctx.image(bench_image(p))
+    xi = ctx.zeros(2 * p.M * p.N)
+    for _ in range(max(warmup, 3) if graphs else max(warmup, 1)):
+        obj.value(d_img)
+        obj.gradient(d_img, xi, 1)
+    ctx.sync()
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        obj.value(d_img)
+        obj.gradient(d_img, xi, 1)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    ms, n = ctx.adjoint_time_ms()
+    obj.close()
+    ctx.close()
+    return {"evals_per_s": steps / el, "ms_per_step": el / steps * 1e3, "steps": steps,
+            "adjoint_kernel_ms": ms if n else None, "split_vis": 3072, "graphs": bool(graphs)}
+
+
+def compact_tti(r: dict) -> dict:
+    """The time-to-image summary with one short record per observation."""
+    obs = []
+    for o in r["observations"]:
+        rec = {"seed_sky": o["seed_sky"], "ratio": o["ratio"],
+               "per_iteration_ratio": o["per_iteration_ratio"],
+               "target_objective": o["target_objective"]}
+        for m, f in o["forms"].items():
+            rec[m] = {k: f[k] for k in ("seconds_to_target", "iterations_to_target",
+                                        "iterations_run", "seconds_run", "final_objective",
+                                        "final_chi2", "split_vis") if k in f}
+            if "image_rel_l2_vs_f32" in f:
+                rec[m]["image_rel_l2_vs_f32"] = f["image_rel_l2_vs_f32"]
+        obs.append(rec)
+    return {k: r[k] for k in ("value", "unit", "config", "median_per_iteration_ratio")} | {
+        "observations": obs}
 
 
 if __name__ == "__main__":

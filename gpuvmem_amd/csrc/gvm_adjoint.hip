@@ -151,6 +151,18 @@ __device__ __forceinline__ void lds_barrier() {
 
 constexpr int kGemmThreads = 512;
 constexpr int kPScale = 15;  // max |P| = 2^15 < 65504 (FP16 max)
+constexpr int kPScaleLimit = 100;
+
+// Exponent of the power-of-two scale of P for a slice whose largest |c| is
+// cm: 2^e cm <= 2^kPScale.  Clamped to [-kPScaleLimit, kPScaleLimit] so that
+// the scale and its inverse stay finite and normal in FP32 for any cm (an
+// unclamped exponent overflows for cm below about 2^-113, and 0 * inf would
+// turn the zero amplitudes of off-grid samples into NaN); 0 when cm is 0.
+__device__ __forceinline__ int p_scale_exponent(float cm) {
+  if (!(cm > 0.0f)) return 0;
+  const int e = kPScale - (int)ceilf(log2f(cm));
+  return min(max(e, -kPScaleLimit), kPScaleLimit);
+}
 constexpr int kHalfKV = kKV / 2;
 
 // One workgroup (8 waves, 2 per SIMD) owns a 256x256 tile of G for one
@@ -211,7 +223,7 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
   // fits FP16; the scale is folded into the visibility amplitudes as they
   // are staged.  Rows and columns past the image produce finite values that
   // only reach outputs past the image, which are not written.
-  const float scale = cm > 0.0f ? exp2f((float)kPScale - ceilf(log2f(cm))) : 1.0f;
+  const float scale = ldexpf(1.0f, p_scale_exponent(cm));
 
   const int lr = lane & 31, lh = lane >> 5;
   const int wr = (wave & 1) * 128, wc = (wave >> 1) * 64;
@@ -469,7 +481,7 @@ __global__ void adjoint_epilogue_kernel(const float* __restrict__ part,
       double inv_scale = 1.0;
       if (scaled) {
         const float cm = __uint_as_float(cmaxs[sl]);
-        inv_scale = cm > 0.0f ? (double)exp2f(ceilf(log2f(cm)) - (float)kPScale) : 1.0;
+        inv_scale = ldexp(1.0, -p_scale_exponent(cm));
       }
       const SliceDev& sd = tab[sl];
       const double inv_nvis = normalize ? 1.0 / (double)sd.nvis : 1.0;
@@ -607,8 +619,16 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
     ctx->epoch++;
   }
   // launch timing (not recorded inside graphs)
-  const bool timed = !ctx->capturing;
-  if (timed) GVM_HIP_TRY(hipEventRecord(ctx->ev_a, ctx->stream));
+  const bool timed = !ctx->capturing && !ctx->timing_paused;
+  int pair = 0;
+  if (timed) {
+    if (ctx->tev_pending == gvm_ctx::kTimingPairs) {
+      gvm_status st = timing_fold(ctx, 1);
+      if (st) return st;
+    }
+    pair = (ctx->tev_head + ctx->tev_pending) % gvm_ctx::kTimingPairs;
+    GVM_HIP_TRY(hipEventRecord(ctx->tev[2 * pair], ctx->stream));
+  }
   if (mode == GVM_ADJOINT_DIRECT) {
     const double dxr = g.deltax * 3.14159265358979323846 / 180.0;
     const double dyr = g.deltay * 3.14159265358979323846 / 180.0;
@@ -635,18 +655,30 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
                          ctx->cmax, ctx->adj_part);
     GVM_CHECK_LAUNCH();
   }
-  if (timed) GVM_HIP_TRY(hipEventRecord(ctx->ev_b, ctx->stream));
+  if (timed) {
+    GVM_HIP_TRY(hipEventRecord(ctx->tev[2 * pair + 1], ctx->stream));
+    ctx->tev_pending++;
+  }
   hipLaunchKernelGGL(adjoint_epilogue_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      ctx->stream, ctx->adj_part, ctx->slice_p0, ctx->slice_tab, nsl, n, ctx->cmax,
                      mode == GVM_ADJOINT_DIRECT ? 0 : 1, ctx->noise, g.noise_cut, g.fg_scale,
                      ctx->gcf, normalize, g.threshold, flag_opt, I, result);
   GVM_CHECK_LAUNCH();
-  if (!timed) return GVM_OK;
-  GVM_HIP_TRY(hipEventSynchronize(ctx->ev_b));
-  float ms = 0.f;
-  GVM_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b));
-  ctx->adj_ms_total += ms;
-  ctx->adj_count += 1;
+  return GVM_OK;
+}
+
+gvm_status timing_fold(gvm_ctx* ctx, int n) {
+  while (ctx->tev_pending > 0 && n != 0) {
+    const int pair = ctx->tev_head;
+    GVM_HIP_TRY(hipEventSynchronize(ctx->tev[2 * pair + 1]));
+    float ms = 0.f;
+    GVM_HIP_TRY(hipEventElapsedTime(&ms, ctx->tev[2 * pair], ctx->tev[2 * pair + 1]));
+    ctx->adj_ms_total += ms;
+    ctx->adj_count += 1;
+    ctx->tev_head = (ctx->tev_head + 1) % gvm_ctx::kTimingPairs;
+    ctx->tev_pending--;
+    if (n > 0) n--;
+  }
   return GVM_OK;
 }
 
@@ -780,8 +812,8 @@ extern "C" gvm_status gvm_autotune_adjoint(gvm_ctx* ctx, const long* candidates,
     set_error("gvm_autotune_adjoint: scratch allocation failed");
     return GVM_ERR_OOM;
   }
-  const double save_ms = ctx->adj_ms_total;
-  const long save_n = ctx->adj_count;
+  // the tuning launches are not part of the adjoint timing
+  ctx->timing_paused = true;
   long best = candidates[0];
   double best_ms = 1e300;
   gvm_status st = GVM_OK;
@@ -806,8 +838,7 @@ extern "C" gvm_status gvm_autotune_adjoint(gvm_ctx* ctx, const long* candidates,
       best = candidates[c];
     }
   }
-  ctx->adj_ms_total = save_ms;
-  ctx->adj_count = save_n;
+  ctx->timing_paused = false;
   if (st) {
     cleanup();
     if (st == GVM_ERR_HIP) set_error("gvm_autotune_adjoint: timing failed");

@@ -309,8 +309,8 @@ extern "C" gvm_status gvm_destroy(gvm_ctx* ctx) {
   if (ctx->hscalars) (void)hipHostFree(ctx->hscalars);
   (void)hipFree(ctx->dvalues);
   if (ctx->hvalues) (void)hipHostFree(ctx->hvalues);
-  if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
-  if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+  for (hipEvent_t& e : ctx->tev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return GVM_OK;
@@ -332,8 +332,7 @@ static gvm_status create_impl(const gvm_geometry* g, const float* noise_host, gv
   }
   ctx->cu_count = prop.multiProcessorCount;
   GVM_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-  GVM_HIP_TRY(hipEventCreate(&ctx->ev_a));
-  GVM_HIP_TRY(hipEventCreate(&ctx->ev_b));
+  for (hipEvent_t& e : ctx->tev) GVM_HIP_TRY(hipEventCreate(&e));
 
   const long n = (long)M * N;
   std::vector<float> noise(noise_host, noise_host + n);
@@ -681,6 +680,8 @@ extern "C" gvm_status gvm_dchi2(gvm_ctx* ctx, const float* I_dev, float* result_
 
 extern "C" gvm_status gvm_timing_reset(gvm_ctx* ctx) {
   GVM_REQUIRE(ctx, "gvm_timing_reset: null context");
+  gvm_status st = gvm::timing_fold(ctx, -1);
+  if (st) return st;
   ctx->adj_ms_total = 0.0;
   ctx->adj_count = 0;
   return GVM_OK;
@@ -688,6 +689,8 @@ extern "C" gvm_status gvm_timing_reset(gvm_ctx* ctx) {
 
 extern "C" gvm_status gvm_timing_adjoint_ms(gvm_ctx* ctx, double* avg_ms, long* count) {
   GVM_REQUIRE(ctx && avg_ms, "gvm_timing_adjoint_ms: null argument");
+  gvm_status st = gvm::timing_fold(ctx, -1);
+  if (st) return st;
   *avg_ms = ctx->adj_count ? ctx->adj_ms_total / ctx->adj_count : 0.0;
   if (count) *count = ctx->adj_count;
   return GVM_OK;

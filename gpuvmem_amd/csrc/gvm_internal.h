@@ -117,13 +117,22 @@ struct gvm_ctx {
   // epoch changes whenever recorded device pointers or launches go stale
   bool capturing = false;
   long epoch = 0;
-  // timing of the adjoint GEMM
-  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  // timing of the adjoint GEMM: a ring of event pairs recorded around each
+  // eager launch and read back lazily (when the ring is full or the timing
+  // is queried), so an evaluation never waits on the host for its timing
+  static constexpr int kTimingPairs = 32;
+  hipEvent_t tev[2 * kTimingPairs] = {};
+  int tev_head = 0, tev_pending = 0;
+  bool timing_paused = false;
   double adj_ms_total = 0.0;
   long adj_count = 0;
 };
 
 namespace gvm {
+
+// Adds the durations of up to n of the oldest pending timed adjoint launches
+// to the totals (waiting for each); n < 0 for all.
+gvm_status timing_fold(gvm_ctx* ctx, int n);
 
 // Launchers implemented in the .hip files.
 gvm_status forward_chi2(gvm_ctx* ctx, float* I, int flag_opt, int normalize,

@@ -72,6 +72,16 @@ void Fi::graphKey(std::vector<int64_t>& key) const {
   key.push_back(key_addr(device_DS));
 }
 
+void Fi::setS(float* S) {
+  if (S != device_S) gvm_free(device_S);
+  device_S = S;
+}
+
+void Fi::setDS(float* DS) {
+  if (DS != device_DS) gvm_free(device_DS);
+  device_DS = DS;
+}
+
 void Fi::configure(int penalizatorIndex, int imageIndex, int imageToAdd, bool normalize) {
   const auto& g = globals();
   this->imageIndex = imageIndex;

@@ -62,19 +62,53 @@ void particularEvaluateXt(float* xt, float* pcom, float* xicom, double x, int im
                         globals().eta),
         "evaluateXt");
 }
+void defaultNewP(float* p, float* xi, float xmin, int image) {
+  defaultNewP(p, xi, (double)xmin, image);
+}
+void particularNewP(float* p, float* xi, float xmin, int image) {
+  particularNewP(p, xi, (double)xmin, image);
+}
+void defaultEvaluateXt(float* xt, float* pcom, float* xicom, float x, int image) {
+  defaultEvaluateXt(xt, pcom, xicom, (double)x, image);
+}
+void particularEvaluateXt(float* xt, float* pcom, float* xicom, float x, int image) {
+  particularEvaluateXt(xt, pcom, xicom, (double)x, image);
+}
 imageMap* makeFunctionMapping(int image_count, bool nopositivity) {
   imageMap* m = new imageMap[image_count];
   for (int i = 0; i < image_count; i++) {
     if (!nopositivity && i == 0) {
       m[i].evaluateXt = particularEvaluateXt;
       m[i].newP = particularNewP;
+      m[i].evaluateXtD = particularEvaluateXt;
+      m[i].newPD = particularNewP;
     } else {
       m[i].evaluateXt = defaultEvaluateXt;
       m[i].newP = defaultNewP;
+      m[i].evaluateXtD = defaultEvaluateXt;
+      m[i].newPD = defaultNewP;
     }
   }
   return m;
 }
+
+namespace {
+// The double-step form of a mapping entry when it has one, else its float
+// form with the step rounded to float (the reference's arithmetic).
+void applyNewP(const imageMap& m, float* p, float* xi, double xmin, int image) {
+  if (m.newPD)
+    m.newPD(p, xi, xmin, image);
+  else
+    m.newP(p, xi, (float)xmin, image);
+}
+void applyEvaluateXt(const imageMap& m, float* xt, float* pcom, float* xicom, double x,
+                     int image) {
+  if (m.evaluateXtD)
+    m.evaluateXtD(xt, pcom, xicom, x, image);
+  else
+    m.evaluateXt(xt, pcom, xicom, (float)x, image);
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // line search
@@ -218,7 +252,7 @@ double brent(double ax, double bx, double cx, double tol, double* xmin, f1dim_fn
 // f(x) = Phi(p + x xi) with the per-image positivity of evaluateXt.
 double f1dim(double x) {
   for (int i = 0; i < I->getImageCount(); i++)
-    (I->getFunctionMapping()[i].evaluateXt)(device_xt, device_pcom, device_xicom, x, i);
+    applyEvaluateXt(I->getFunctionMapping()[i], device_xt, device_pcom, device_xicom, x, i);
   return testof->calcFunctionD(device_xt);
 }
 
@@ -253,7 +287,8 @@ void linmin(float* p, float* xi, double* fret, f1dim_fn func) {
     *fret = brent(ax, xx, bx, 1.0e-7, &xmin, f1dim);
   }
   if (globals().verbose) std::printf("Alpha for linear minimization = %f\n\n", xmin);
-  for (int i = 0; i < I->getImageCount(); i++) (I->getFunctionMapping()[i].newP)(p, xi, xmin, i);
+  for (int i = 0; i < I->getImageCount(); i++)
+    applyNewP(I->getFunctionMapping()[i], p, xi, xmin, i);
 }
 
 // Line-search buffers are allocated once per optimize() (the reference

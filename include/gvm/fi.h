@@ -13,6 +13,11 @@
 
 class CKernel;  // gridding kernels are not part of this path
 
+// The reference's cufftComplex (8 bytes, {x, y} = {re, im}).
+typedef struct gvm_complex {
+  float x, y;
+} gvm_complex;
+
 class Fi {
  public:
   Fi();
@@ -54,6 +59,14 @@ class Fi {
   float getPenalizationFactor() { return this->penalization_factor; }
   void set_fivalue(double fi) { this->fi_value = fi; }
   void setPenalizationFactor(float p) { this->penalization_factor = p; }
+  // include/classes/fi.cuh:44-53.  Inu is a device pointer the term keeps
+  // for its caller (the terms here do not read it).  setS / setDS hand the
+  // term a device buffer of M*N floats for its values / gradient: the term
+  // frees the one it held (gvm_free) and owns the new one.
+  void setInu(gvm_complex* Inu) { this->Inu = Inu; }
+  gvm_complex* getInu() { return this->Inu; }
+  void setS(float* S);
+  void setDS(float* DS);
   void setIteration(int iteration) { this->iteration = iteration; }
   int getIteration() { return this->iteration; }
   void setNormalize(bool normalize) { this->normalize = normalize; }
@@ -71,6 +84,7 @@ class Fi {
   int imageIndex = 0;
   int iteration = 0;
   std::string name;
+  gvm_complex* Inu = nullptr;
   int imageToAdd = 0;
   bool normalize = false;
 };

@@ -4,9 +4,18 @@
 // (src/functions.cu:4566-4592, src/mfs.cu:794-811).
 #pragma once
 
+// newP and evaluateXt keep the reference's float step (image.cuh:4-7), so a
+// mapping filled the reference's way (`m.newP = particularNewP;`, which picks
+// the float overload below) works unchanged.  newPD and evaluateXtD are
+// additions: the same functions with the step as a double, which is what
+// Brent's minimisation returns here.  The optimisers call the double forms
+// when they are set and the float forms (with the step rounded to float, as
+// in the reference) otherwise; makeFunctionMapping sets both.
 typedef struct functionMap {
-  void (*newP)(float*, float*, double, int);
-  void (*evaluateXt)(float*, float*, float*, double, int);
+  void (*newP)(float*, float*, float, int);
+  void (*evaluateXt)(float*, float*, float*, float, int);
+  void (*newPD)(float*, float*, double, int) = nullptr;
+  void (*evaluateXtD)(float*, float*, float*, double, int) = nullptr;
 } imageMap;
 
 class Image {
@@ -29,7 +38,12 @@ class Image {
 };
 
 // src/functions.cu:4566-4592: positivity on image 0 (floor -eta*MINPIX),
-// none on the others.
+// none on the others.  Float-step overloads (the reference's signatures) and
+// double-step ones.
+void defaultNewP(float* p, float* xi, float xmin, int image);
+void particularNewP(float* p, float* xi, float xmin, int image);
+void defaultEvaluateXt(float* xt, float* pcom, float* xicom, float x, int image);
+void particularEvaluateXt(float* xt, float* pcom, float* xicom, float x, int image);
 void defaultNewP(float* p, float* xi, double xmin, int image);
 void particularNewP(float* p, float* xi, double xmin, int image);
 void defaultEvaluateXt(float* xt, float* pcom, float* xicom, double x, int image);

@@ -27,6 +27,14 @@ static inline double gcf_at(const orc_geom* g, long k) {
   return g->gcf ? (double)g->gcf[k] : 1.0;
 }
 
+/* Thread count of the OpenMP regions that follow (the environment's
+ * OMP_NUM_THREADS is read once, when the OpenMP runtime loads). */
+void orc_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
 /* sin/cos of 2*pi*t with the argument reduced exactly in turns first. */
 static inline void sincos_turns(double t, double* s, double* c) {
   double r = t - nearbyint(t);
@@ -389,6 +397,37 @@ void orc_dchi2_direct_pixels(const orc_geom* g, const orc_beam* b,
     if (normalize) d /= (double)ch->nvis;
     out[p] = -d;
   }
+}
+
+/* orc_dchi2 (direct sums, every channel, chain rule) at sampled pixels only:
+ * out[2 p] = result[0] and out[2 p + 1] = result[1] of pixel p.  The
+ * chain rule of DChi2_total_I_nu_0 / DChi2_total_alpha
+ * (src/functions.cu:3968-4024), accumulated channel after channel as in
+ * orc_dchi2. */
+void orc_dchi2_pixels(const orc_geom* g, const orc_beam* b, const orc_chan* ch, int nch,
+                      const double* I, const ocplx* vr, int flag_opt, int normalize,
+                      const int* pix, int npix, double* out) {
+  long n = (long)g->M * g->N;
+  double* d = (double*)malloc(sizeof(double) * (size_t)(npix > 0 ? npix : 1));
+  for (int p = 0; p < 2 * npix; p++) out[p] = 0.0;
+  long off = 0;
+  for (int c = 0; c < nch; c++) {
+    if (ch[c].nvis <= 0) continue;
+    orc_dchi2_direct_pixels(g, b, &ch[c], vr + off, normalize, pix, npix, d);
+    off += ch[c].nvis;
+    double nudiv = (double)ch[c].freq / (double)g->nu_0;
+    for (int p = 0; p < npix; p++) {
+      long k = (long)g->N * pix[2 * p] + pix[2 * p + 1];
+      if (!(g->noise[k] < g->noise_cut)) continue;
+      double i0 = I[k], alpha = I[n + k];
+      double dI = pow(nudiv, alpha);
+      if (flag_opt % 2 == 0)
+        out[2 * p] += d[p] * dI;
+      else if (i0 > g->threshold)
+        out[2 * p + 1] += d[p] * i0 * dI * (double)g->fg_scale * log(nudiv);
+    }
+  }
+  free(d);
 }
 
 /* Separable evaluation of the same sum.  exp(2 pi i (x u + y v + (z-1) w))

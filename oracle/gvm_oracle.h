@@ -116,9 +116,17 @@ double orc_chi2(const orc_geom* g, const orc_beam* b, orc_chan* ch, int nch,
 void orc_dchi2_channel(const orc_geom* g, const orc_beam* b,
                        const orc_chan* ch, const ocplx* vr, int normalize,
                        int separable, double* dchi2);
+/* omp_set_num_threads for the calling thread's later parallel regions. */
+void orc_set_threads(int n);
+int orc_max_threads(void);
 void orc_dchi2_direct_pixels(const orc_geom* g, const orc_beam* b,
                              const orc_chan* ch, const ocplx* vr, int normalize,
                              const int* pix, int npix, double* out);
+/* orc_dchi2 with direct sums at sampled pixels (pix: npix (row, col) pairs);
+ * out: npix x 2 (I_nu_0 and alpha components). */
+void orc_dchi2_pixels(const orc_geom* g, const orc_beam* b, const orc_chan* ch, int nch,
+                      const double* I, const ocplx* vr, int flag_opt, int normalize,
+                      const int* pix, int npix, double* out);
 void orc_dchi2(const orc_geom* g, const orc_beam* b, const orc_chan* ch,
                int nch, const double* I, const ocplx* vr, double* result,
                int flag_opt, int normalize, int separable);

@@ -16,12 +16,13 @@ def conv_fwd(x, w, b, relu):
     Co, _, K, _ = w.shape
     P = K // 2
     xp = np.pad(x, ((0, 0), (0, 0), (P, P), (P, P)))
-    out = np.zeros((B, Co, H, W))
+    out = np.zeros((B, Co, H * W))
     for ky in range(K):
         for kx in range(K):
-            patch = xp[:, :, ky:ky + H, kx:kx + W]  # [B,Ci,H,W]
-            out += np.einsum("oc,bchw->bohw", w[:, :, ky, kx], patch)
-    out += b[None, :, None, None]
+            patch = xp[:, :, ky:ky + H, kx:kx + W].reshape(B, Ci, H * W)
+            # [Co,Ci] @ [B,Ci,HW]; contiguous operands so that numpy uses BLAS
+            out += np.matmul(np.ascontiguousarray(w[:, :, ky, kx]), patch)
+    out = out.reshape(B, Co, H, W) + b[None, :, None, None]
     return np.maximum(out, 0.0) if relu else out
 
 
@@ -34,11 +35,14 @@ def conv_bwd(x, w, out, dout, relu):
     xp = np.pad(x, ((0, 0), (0, 0), (P, P), (P, P)))
     dw = np.zeros_like(w, dtype=np.float64)
     dxp = np.zeros_like(xp, dtype=np.float64)
+    g2 = np.ascontiguousarray(g, np.float64).reshape(B, Co, H * W)
     for ky in range(K):
         for kx in range(K):
-            patch = xp[:, :, ky:ky + H, kx:kx + W]
-            dw[:, :, ky, kx] = np.einsum("bohw,bchw->oc", g, patch)
-            dxp[:, :, ky:ky + H, kx:kx + W] += np.einsum("oc,bohw->bchw", w[:, :, ky, kx], g)
+            patch = xp[:, :, ky:ky + H, kx:kx + W].reshape(B, Ci, H * W)
+            dw[:, :, ky, kx] = np.matmul(g2, patch.transpose(0, 2, 1)).sum(axis=0)
+            wt = np.ascontiguousarray(w[:, :, ky, kx].T)
+            dxp[:, :, ky:ky + H, kx:kx + W] += np.matmul(wt, g2).reshape(
+                B, Ci, H, W)
     db = g.sum(axis=(0, 2, 3))
     dx = dxp[:, :, P:P + H, P:P + W]
     return dx, dw, db

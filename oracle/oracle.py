@@ -108,6 +108,9 @@ def lib() -> C.CDLL:
             "orc_dchi2_channel": ([GP, BP, CP, P, I, I, P], None),
             "orc_dchi2_direct_pixels": ([GP, BP, CP, P, I, P, I, P], None),
             "orc_dchi2": ([GP, BP, CP, I, P, P, P, I, I, I], None),
+            "orc_dchi2_pixels": ([GP, BP, CP, I, P, P, I, I, P, I, P], None),
+            "orc_set_threads": ([I], None),
+            "orc_max_threads": ([], I),
             "orc_entropy_value": ([GP, P, I, D, D], D),
             "orc_entropy_grad": ([GP, P, I, D, D, D, P], None),
             "orc_tsv_value": ([GP, P, I], D),
@@ -306,6 +309,18 @@ class Oracle:
                                           _ptr(out))
         return out
 
+    def dchi2_pixels_all(self, image, vr, pixels, flag_opt: int = 0, normalize: bool = False):
+        """dchi2 over every channel (direct sums, chain rule to I_nu_0 or
+        alpha) at sampled pixels: array [npix, 2]."""
+        img = np.ascontiguousarray(image, np.float64).reshape(2, self.p.N, self.p.M)
+        vr = np.ascontiguousarray(vr, np.complex128)
+        pix = np.ascontiguousarray(np.asarray(pixels, np.int32).reshape(-1, 2))
+        out = np.zeros((pix.shape[0], 2), np.float64)
+        self._lib.orc_dchi2_pixels(C.byref(self.geom), C.byref(self.beam), self.chan, self.nch,
+                                   _ptr(img), _ptr(vr), flag_opt, int(normalize), _ptr(pix),
+                                   pix.shape[0], _ptr(out))
+        return out
+
     # ---- priors ----
     def prior_value(self, kind: str, image, index: int = 0, prior: float = 0.001,
                     eta: float = -1.0) -> float:
@@ -453,4 +468,7 @@ class PyProblem:
 
 
 def set_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP loops from now on (also for a runtime
+    that was loaded before, when OMP_NUM_THREADS is no longer read)."""
     os.environ["OMP_NUM_THREADS"] = str(n)
+    lib().orc_set_threads(n)

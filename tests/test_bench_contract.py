@@ -21,7 +21,7 @@ PARTIAL_BYTES = RANGES * 1024 * 1024 * 4
 
 @pytest.mark.parametrize("mode", ["bf16x3", "f16x3"])
 def test_traffic_summary_for_each_mfma_form(mode):
-    traffic, source = bench.profiled_traffic(mode, NVIS)
+    traffic, source = bench.profiled_traffic(mode, NVIS, 3072)
     assert source is not None, f"no counter summary for {mode} at {NVIS} visibilities"
     assert (ROOT / "profiles" / source).exists()
     # at least the partial tiles it writes, at most a few percent more than that plus reads
@@ -30,8 +30,17 @@ def test_traffic_summary_for_each_mfma_form(mode):
     assert f"--adjoint {mode}" in text or mode == "bf16x3"
 
 
-def test_no_summary_for_an_unprofiled_size():
-    assert bench.profiled_traffic("bf16x3", NVIS + 1) == (None, None)
+def test_no_summary_for_an_unprofiled_size_or_range_length():
+    assert bench.profiled_traffic("bf16x3", NVIS + 1, 3072) == (None, None)
+    # another range length writes another number of partial images
+    assert bench.profiled_traffic("bf16x3", NVIS, 1536) == (None, None)
+
+
+def test_host_cpu_facts():
+    hc = bench.host_cpus()
+    assert hc["nproc"] >= 1 and 1 <= hc["threads"] <= hc["nproc"]
+    if hc["cgroup_cpu_quota"] is not None:
+        assert hc["threads"] <= max(1, int(hc["cgroup_cpu_quota"]))
 
 
 def test_more_than_one_gpu_is_refused():

@@ -70,3 +70,15 @@ def test_composition_matches_flat_entry(tmp_path, optimizer):
     traj = np.array(traj)
     print(f"C++: {traj_cpp}\nflat: {traj}")
     assert len(traj_cpp) >= 2 and np.array_equal(traj_cpp, traj)
+    # the mapping filled the reference's way (float steps only): the same
+    # first value, and then values that differ only by the float rounding of
+    # each accepted step (relative 6e-8 on the step)
+    r = subprocess.run([str(exe), str(inp), optimizer, str(iters), "float-steps"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    line = next(l for l in r.stdout.splitlines() if l.startswith("trajectory"))
+    traj_f = np.array([float(v) for v in line.split()[1:]])
+    n = min(len(traj_f), len(traj))
+    rel = np.abs(traj_f[:n] - traj[:n]) / np.abs(traj[:n])
+    print(f"float steps: {traj_f}; max rel {rel.max():.2e}")
+    assert n >= 2 and rel.max() < 1e-3

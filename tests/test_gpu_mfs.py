@@ -61,7 +61,7 @@ def test_mfs_batched_parity(mode):
     for flag in (0, 1):
         err = _check(p, orc, mode, img, flag)
         print(f"flag {flag}: gradient rel L2 {err:.3e}")
-        assert err < (GRAD_TOL if mode != "bf16x3" else 3e-5), (mode, flag, err)
+        assert err < GRAD_TOL, (mode, flag, err)
 
 
 def test_mfs_many_slices_and_normalize():

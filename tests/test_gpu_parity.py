@@ -51,9 +51,8 @@ def test_chi2_matches_oracle(prob256, orc256):
     ctx.close()
 
 
-@pytest.mark.parametrize("mode,tol", [("direct", GRAD_TOL), ("f32", GRAD_TOL), ("f16x3", GRAD_TOL),
-                                      ("bf16x3", 3e-4)])
-def test_dchi2_matches_oracle(prob256, orc256, mode, tol):
+@pytest.mark.parametrize("mode", ["direct", "f32", "f16x3", "bf16x3"])
+def test_dchi2_matches_oracle(prob256, orc256, mode):
     p = prob256
     print(f"case: dchi2 256^2 mode={mode}", flush=True)
     ctx = Context(p, adjoint=mode)
@@ -68,6 +67,6 @@ def test_dchi2_matches_oracle(prob256, orc256, mode, tol):
     g_ref = orc256.dchi2(clipped, vr_ref)
     err = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
     print(f"mode={mode} rel L2 {err:.3e}")
-    assert err < tol
+    assert err < GRAD_TOL
     assert np.all(g[1] == 0.0)  # flag_opt 0: no alpha gradient
     ctx.close()

@@ -233,6 +233,54 @@ def test_dchi2_chain_rule_flags(prob, orc):
     assert np.all(r1[0] == 0) and np.allclose(r1[1], 0.0)
 
 
+def test_dchi2_two_channel_chain_rule_at_sampled_pixels():
+    """Two channels at different frequencies: the sampled-pixel gradient (and
+    the whole-image one) equal the chain rule written out here from each
+    channel's own dchi2: sum_c d_c (nu_c/nu0)^alpha for I_nu_0 (even flag);
+    sum_c d_c I0 (nu_c/nu0)^alpha fg ln(nu_c/nu0) where I0 > threshold for
+    alpha (odd flag), src/functions.cu:3968-4024."""
+    p = small_problem(npix=32, n_antennas=5, seed=3)
+    q = small_problem(npix=32, n_antennas=5, seed=3, freq_hz=245e9, seed_noise=77)
+    slices = [(p.freq, p.uvw, p.vo, p.weight), (q.freq, q.uvw, q.vo, q.weight)]
+    p.threshold = 20.0
+    p.noise = p.noise.copy()
+    p.noise[0, :2] = 2 * p.noise_cut  # two pixels outside the mask
+    orc = oracle.Oracle(p, slices=slices)
+    rng = np.random.default_rng(8)
+    img = p.initial_image().astype(np.float64)
+    img[0] += rng.random((p.N, p.M)) * 40
+    img[1] = rng.standard_normal((p.N, p.M)) * 0.7
+    inside = np.argwhere(p.noise < p.noise_cut)
+    pix = inside[rng.choice(len(inside), 12, replace=False)]
+    pix = np.vstack([pix, np.argwhere(p.noise >= p.noise_cut)[:2]])
+    for flag in (0, 1, 2, 3):
+        _, vr, clipped = orc.chi2(img, flag_opt=flag)
+        whole = orc.dchi2(clipped, vr, flag_opt=flag, separable=False)
+        got = orc.dchi2_pixels_all(clipped, vr, pix, flag_opt=flag)
+        hand = np.zeros((len(pix), 2))
+        off = 0
+        for f, u, v, w in slices:
+            one = oracle.Oracle(p, slices=[(f, u, v, w)])
+            n = u.shape[0]
+            d = one.dchi2_pixels(vr[off:off + n], pix)
+            off += n
+            r = f / p.nu_0
+            for k, (i, j) in enumerate(pix):
+                if not p.noise[i, j] < p.noise_cut:
+                    continue
+                i0, a = clipped[0, i, j], clipped[1, i, j]
+                if flag % 2 == 0:
+                    hand[k, 0] += d[k] * r ** a
+                elif i0 > p.threshold:
+                    hand[k, 1] += d[k] * i0 * r ** a * p.fg_scale * np.log(r)
+        ref = np.array([[whole[0, i, j], whole[1, i, j]] for i, j in pix])
+        assert np.allclose(got, ref, rtol=1e-12, atol=0.0), flag
+        assert np.allclose(got, hand, rtol=1e-12, atol=0.0), flag
+        k = flag % 2
+        assert np.all(got[:, 1 - k] == 0) and np.any(got[:, k] != 0)
+        assert np.all(got[-2:] == 0)  # outside the mask
+
+
 # ---------------------------------------------------------------- priors (a12-a14)
 def _const(prob, c):
     img = np.zeros((2, prob.N, prob.M))
