@@ -161,6 +161,10 @@ gvm_status register_slice(gvm_ctx* ctx, Slice& s);
 gvm_status primary_beam_dev(const gvm_geometry& g, double freq, float* out, hipStream_t stream);
 gvm_status read_scalars(gvm_ctx* ctx, int nslots, double* out);
 int reduce_grid(long n);
+// Sums (or takes the max of) nblocks FP64 partials per slot, slot s reading
+// partials[s * kReduceBlocks ...], in a fixed order (gvm_ctx.hip).
+__global__ void finish_kernel(const double* __restrict__ partials, double* __restrict__ out,
+                              int nblocks, int is_max);
 
 // Loads and stores through global-address-space pointers.  Pointers read
 // from device tables (the slice descriptors) are generic, and generic (flat)

@@ -9,6 +9,9 @@
 // FP64 here.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "gvm_internal.h"
 
 namespace gvm {
@@ -461,6 +464,62 @@ extern "C" gvm_status gvm_vec_dot_dev(gvm_ctx* ctx, const float* a, const float*
   hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, ctx->stream, a, b, n, ctx->partials);
   GVM_CHECK_LAUNCH();
   return finish_reductions_to(ctx, 1, nb, out_dev);
+}
+
+// ---------------------------------------------------------------------------
+// gvm_vec_dot_dev without a context and without allocation.  The block
+// partials live in `scratch`, a static device array of kReduceBlocks doubles
+// (8 KiB) that every GPU running this library holds until the process ends.
+// Host code never passes `scratch` itself to a kernel: on the host the name
+// is only a stand-in, so its device address on the current GPU is looked up
+// once per device (hipGetSymbolAddress) and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kReduceBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+gvm_status scratch_on_current_device(double** out) {
+  int dev = 0;
+  GVM_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("gvm_vec_dot_dev_noalloc: device index beyond the library's table");
+    return GVM_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("gvm_vec_dot_dev_noalloc: hipGetSymbolAddress(scratch): ") +
+                hipGetErrorString(e));
+      return GVM_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GVM_OK;
+}
+}  // namespace
+
+extern "C" gvm_status gvm_vec_dot_dev_noalloc(const float* a, const float* b, long n,
+                                              double* out_dev, void* stream) {
+  GVM_REQUIRE(a && b && out_dev, "gvm_vec_dot_dev_noalloc: null argument");
+  GVM_REQUIRE(n > 0 && n <= GVM_NOALLOC_DOT_MAX_N,
+              "gvm_vec_dot_dev_noalloc: n must be in [1, GVM_NOALLOC_DOT_MAX_N]");
+  double* part = nullptr;
+  gvm_status st = scratch_on_current_device(&part);
+  if (st) return st;
+  const hipStream_t s = (hipStream_t)stream;
+  // the kernels and launch shapes of gvm_vec_dot_dev: the same sum, bit for bit
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, s, a, b, n, part);
+  GVM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(kReduceThreads), 0, s, part, out_dev, nb, 0);
+  GVM_CHECK_LAUNCH();
+  return GVM_OK;
 }
 
 extern "C" gvm_status gvm_scalar_rho_times(gvm_ctx* ctx, const double* rho_den, const double* dot,

@@ -13,6 +13,8 @@ LIB_DIR = Path(__file__).resolve().parent / "lib"
 
 GVM_OK = 0
 GVM_ERR_ARG = 1
+# include/gvm_hip.h: largest n of gvm_vec_dot_dev_noalloc
+GVM_NOALLOC_DOT_MAX_N = 1 << 37
 GVM_BEAM_AIRYDISK = 0
 GVM_BEAM_GAUSSIAN = 1
 ADJOINT_MODES = {"f32": 0, "f16x3": 1, "bf16x3": 2, "direct": 3}
@@ -190,6 +192,7 @@ HIP_API = {
     "gvm_vec_dot": [_P, _P, _P, _L, _DP],
     "gvm_vec_axpy": [_P, _P, _P, _D, _L],
     "gvm_vec_dot_dev": [_P, _P, _P, _L, _P],
+    "gvm_vec_dot_dev_noalloc": [_P, _P, _L, _P, _P],
     "gvm_scalar_rho_times": [_P, _P, _P, _P, _P],
     "gvm_scalar_h0": [_P, _P, _P, _I, _P],
     "gvm_vec_axpy_dev": [_P, _P, _P, _P, _I, _L],

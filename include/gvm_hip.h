@@ -373,6 +373,20 @@ gvm_status gvm_vec_axpy(gvm_ctx* ctx, float* y, const float* x, double alpha,
  *   gvm_vec_axpy_dev:     y += (float)(negate ? -*coef : *coef) * x;
  *   gvm_vec_scale_dev:    dst = src * (float)*coef. */
 gvm_status gvm_vec_dot_dev(gvm_ctx* ctx, const float* a, const float* b, long n, double* out_dev);
+/* gvm_vec_dot_dev with no context and no allocation (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc.  The
+ * FP64 block partials live in a static device array of 1024 doubles (8 KiB)
+ * that each GPU the library runs on holds until the process ends; its device
+ * address is looked up on the first call on each device.  The result equals
+ * gvm_vec_dot_dev's bit for bit.  Enqueued on `stream` (a hipStream_t; NULL:
+ * the null stream), nothing waits.  Calls that overlap on one GPU (other
+ * streams, other threads) share the array and race; the partials need no
+ * clearing (each call writes every partial it reads).  n in
+ * [1, GVM_NOALLOC_DOT_MAX_N], otherwise GVM_ERR_ARG and nothing is launched. */
+#define GVM_NOALLOC_DOT_MAX_N (1L << 37)
+gvm_status gvm_vec_dot_dev_noalloc(const float* a, const float* b, long n, double* out_dev,
+                                   void* stream);
 gvm_status gvm_scalar_rho_times(gvm_ctx* ctx, const double* rho_den, const double* dot,
                                 const double* minus_from, double* out);
 gvm_status gvm_scalar_h0(gvm_ctx* ctx, const double* sy, const double* yy, int nimg,
