@@ -368,13 +368,14 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
     // main loop: one barrier per chunk
     for (int c = 0; c < nchunks; c++) {
       const int cur = c & 1, nxt = cur ^ 1;
-      if (!group_b) {
-        generate(nxt);  // chunk c+1 (past the end: zero weights, never multiplied)
-        multiply(cur);
-      } else {
-        multiply(cur);
-        generate(nxt);
-      }
+      // chunk c+1 (past the end: zero weights, never multiplied) is generated
+      // before or after the multiply by the wave's group.  The multiply is one
+      // instance outside the branches, so the accumulators keep one register
+      // assignment through the loop (two instances, one per branch, cost
+      // about 4 % in copies between them).
+      if (!group_b) generate(nxt);
+      multiply(cur);
+      if (group_b) generate(nxt);
       if (wave == 0) {
         store_rec(cur);  // records of chunk c+2
         load_rec(c + 3);
@@ -396,6 +397,24 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
         const int c = tj * kTile + wc + cb * 32 + lr;
         if (r < N && c < N) out[(long)r * N + c] = acc[rb][cb][e];
       }
+}
+
+// c = w conj(V_r) of every visibility of slice blockIdx.y in polar form, (|c|,
+// arg(c) in turns), from the residuals of the last forward pass: the GEMM's
+// per-visibility input.  |c| is computed as the degrid kernel computes the
+// per-slice max that sets the P scale.
+__global__ __launch_bounds__(256) void polar_kernel(const SliceDev* __restrict__ tab) {
+  const SliceDev& sd = tab[blockIdx.y];
+  const float2* __restrict__ vr = sd.vr;
+  const float* __restrict__ weight = sd.weight;
+  float2* __restrict__ cpol = sd.cpol;
+  const long nvis = sd.nvis;
+  for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < nvis;
+       v += (long)gridDim.x * blockDim.x) {
+    const float2 r = ldg(vr + v);
+    const float w = ldg(weight + v);
+    stg(cpol + v, make_float2(w * sqrtf(resid_abs2(r)), -atan2f(r.y, r.x) * 0.15915494309189535f));
+  }
 }
 
 // Direct sum (the reference's algorithm, exact w term), one thread per pixel,
@@ -617,6 +636,13 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
     gvm_status st = build_splits(ctx, mode, ntiles, n);
     if (st) return st;
     ctx->epoch++;
+  }
+  if (mode != GVM_ADJOINT_DIRECT) {
+    long vmax = 0;
+    for (const Slice& s : ctx->slices) vmax = std::max(vmax, s.nvis);
+    const unsigned nbp = (unsigned)std::min<long>(2048, std::max<long>(1, (vmax + 255) / 256));
+    hipLaunchKernelGGL(polar_kernel, dim3(nbp, nsl), dim3(256), 0, ctx->stream, ctx->slice_tab);
+    GVM_CHECK_LAUNCH();
   }
   // launch timing (not recorded inside graphs)
   const bool timed = !ctx->capturing && !ctx->timing_paused;

@@ -7,16 +7,20 @@
 // covers all slices of the context in one launch (blockIdx.y = slice, one uv
 // grid per slice, per-slice descriptors in a device table):
 //
-//  1. clip_kernel        clip2IWNoise in place (once per evaluation);
-//  2. fft_pass<rows>     I_nu = clip(I0 (nu/nu0)^alpha), times PB and fg_scale,
-//                        then the row transforms (Stockham radix-4/2 in LDS),
-//                        times the column-frequency phase-rotation factor;
-//  3. fft_pass<cols>     column transforms in LDS, times the row-frequency
+//  1. fft_pass<rows>     clip2IWNoise in place on the caller's image, then
+//                        I_nu = clip(I0 (nu/nu0)^alpha), times PB and
+//                        fg_scale, then the row transforms (Stockham
+//                        radix-4/2 in LDS), times the column-frequency
+//                        phase-rotation factor;
+//  2. fft_pass<cols>     column transforms in LDS, times the row-frequency
 //                        phase-rotation factor, in place on the grid;
-//  4. degrid_chi2        bilinear 4-tap gather, V_r = V_o - V_m, w |V_r|^2
+//  3. degrid_chi2        bilinear 4-tap gather, V_r = V_o - V_m, w |V_r|^2
 //                        summed per block in FP64, and max |w V_r| for the
 //                        adjoint's FP16 scaling;
-//  5. finish_kernel      fixed-order FP64 sum of the block partials.
+//  4. finish             fixed-order FP64 sum of the block partials.
+// The polar form of c = w conj(V_r) that the adjoint reads is written by the
+// adjoint itself (polar_kernel), so value-only evaluations (line-search
+// probes) do not pay for it.
 //
 // The transform is e^{+2 pi i} (CUFFT_INVERSE) and unnormalised.  The phase
 // rotation e^{-2 pi i (u x0/M + v y0/N)} (src/functions.cu:2483-2518)
@@ -30,24 +34,40 @@
 namespace gvm {
 
 // ---------------------------------------------------------------------------
-// clip2IWNoise: src/functions.cu:2694-2719
+// clip2IWNoise: src/functions.cu:2694-2719, done by the row pass as it reads
+// the image (one pass and one launch less per evaluation)
 // ---------------------------------------------------------------------------
-// Block 0 also clears the per-slice max |c| that the degrid kernel raises
-// (one launch less per evaluation).
-__global__ void clip_kernel(float* __restrict__ I, const float* __restrict__ noise, long n,
-                            float noise_cut, float floor_v, int eta_pos, float threshold,
-                            int flag_opt, unsigned int* __restrict__ cmax, int nsl) {
-  if (blockIdx.x == 0)
-    for (int s = threadIdx.x; s < nsl; s += blockDim.x) cmax[s] = 0u;
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
-       k += (long)gridDim.x * blockDim.x) {
-    if (noise[k] > noise_cut) {
-      I[k] = eta_pos ? 0.0f : floor_v;
-      I[n + k] = 0.0f;
-    } else if (I[k] < threshold && flag_opt > 0) {
-      I[n + k] = 0.0f;
+struct ClipArgs {
+  const float* noise;
+  unsigned int* cmax;  // per-slice max |c|, cleared here for the degrid kernel
+  float noise_cut;
+  float threshold;
+  int eta_pos;
+  int flag_opt;
+  int nsl;
+};
+
+// The clipped (I0, alpha) of pixel k; writes them back to the caller's image
+// when `store` (one block per pixel does).  Blocks of other slices read the
+// same pixel and may see it before or after that store: the clip is
+// idempotent, so every block computes the same values.
+__device__ __forceinline__ void clip_pixel(float* __restrict__ I, long n, long k,
+                                           const ClipArgs& c, float floor_v, bool store,
+                                           float* i0, float* alpha) {
+  float a = I[k], b = I[n + k];
+  if (c.noise[k] > c.noise_cut) {
+    a = c.eta_pos ? 0.0f : floor_v;
+    b = 0.0f;
+    if (store) {
+      I[k] = a;
+      I[n + k] = b;
     }
+  } else if (a < c.threshold && c.flag_opt > 0) {
+    b = 0.0f;
+    if (store) I[n + k] = b;
   }
+  *i0 = a;
+  *alpha = b;
 }
 
 // ---------------------------------------------------------------------------
@@ -157,13 +177,14 @@ __device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ 
   }
 }
 
-// Row pass: build I_nu * PB * fg_scale [* GCF] for B rows of slice blockIdx.y
+// Row pass: clip2IWNoise on the rows' pixels (written back by the slice-0
+// blocks), then I_nu * PB * fg_scale [* GCF] for B rows of slice blockIdx.y
 // (calculateInu + apply_beam2I, src/functions.cu:3939-3966, 2424-2444),
 // transform each row, multiply by rot_x[column frequency], write the rows of
 // that slice's grid.
 template <int N, int B, int T>
 __global__ __launch_bounds__(T) void fft_rows_kernel(
-    const float* __restrict__ I, const SliceDev* __restrict__ tab, float fg_scale,
+    float* __restrict__ I, ClipArgs clip, const SliceDev* __restrict__ tab, float fg_scale,
     float floor_v, const float* __restrict__ gcf, const float2* __restrict__ tw,
     const float2* __restrict__ rot_x, float2* __restrict__ grids) {
   __shared__ float2 lds[B * N];
@@ -172,9 +193,13 @@ __global__ __launch_bounds__(T) void fft_rows_kernel(
   const float nudiv = tab[blockIdx.y].nudiv;
   float2* __restrict__ grid = grids + (long)blockIdx.y * n;
   const int row0 = blockIdx.x * B;
+  if (blockIdx.x == 0 && blockIdx.y == 0)
+    for (int s = threadIdx.x; s < clip.nsl; s += T) clip.cmax[s] = 0u;
   for (int idx = threadIdx.x; idx < B * N; idx += T) {
     const long k = (long)(row0 + idx / N) * N + (idx % N);
-    float inu = I[k] * powf(nudiv, I[n + k]);
+    float i0, alpha;
+    clip_pixel(I, n, k, clip, floor_v, blockIdx.y == 0, &i0, &alpha);
+    float inu = i0 * powf(nudiv, alpha);
     if (inu < floor_v) inu = floor_v;
     float x = inu * pb[k] * fg_scale;
     if (gcf) x = x * gcf[k];  // apply_GCF (src/functions.cu:2468-2476)
@@ -216,9 +241,9 @@ __global__ __launch_bounds__(T) void fft_cols_kernel(float2* __restrict__ grids,
 }
 
 template <int N, int BR, int TR, int BC, int TC>
-gvm_status launch_fft_with(gvm_ctx* ctx, const float* I, int nsl, float floor_v) {
+gvm_status launch_fft_with(gvm_ctx* ctx, float* I, const ClipArgs& clip, int nsl, float floor_v) {
   hipLaunchKernelGGL((fft_rows_kernel<N, BR, TR>), dim3(N / BR, nsl), dim3(TR), 0, ctx->stream, I,
-                     ctx->slice_tab, ctx->g.fg_scale, floor_v, ctx->gcf, ctx->twiddle,
+                     clip, ctx->slice_tab, ctx->g.fg_scale, floor_v, ctx->gcf, ctx->twiddle,
                      ctx->rot_x, ctx->grid);
   GVM_CHECK_LAUNCH();
   hipLaunchKernelGGL((fft_cols_kernel<N, BC, TC>), dim3(N / BC, nsl), dim3(TC), 0, ctx->stream,
@@ -234,7 +259,7 @@ gvm_status launch_fft_with(gvm_ctx* ctx, const float* I, int nsl, float floor_v)
 // CUs.  Automatic choice (ctx->fft_mode 0): narrow when the wide row pass
 // would launch fewer than 512 blocks.
 template <int N>
-gvm_status launch_fft(gvm_ctx* ctx, const float* I, int nsl, float floor_v) {
+gvm_status launch_fft(gvm_ctx* ctx, float* I, const ClipArgs& clip, int nsl, float floor_v) {
   constexpr int BR = (4096 / N) > 0 ? (4096 / N) : 1;
   constexpr int BC0 = 8192 / N;
   constexpr int BC = BC0 < 2 ? 2 : (BC0 > N ? N : BC0);
@@ -244,22 +269,26 @@ gvm_status launch_fft(gvm_ctx* ctx, const float* I, int nsl, float floor_v) {
     constexpr int BCn = 256 / N > 2 ? 256 / N : 2;
     constexpr int TCn = BCn * N / 4 < 256 ? BCn * N / 4 : 256;
     const bool narrow = ctx->fft_mode == 2 || (ctx->fft_mode == 0 && (long)nsl * (N / BR) < 512);
-    if (narrow) return launch_fft_with<N, BRn, TRn, BCn, TCn>(ctx, I, nsl, floor_v);
+    if (narrow) return launch_fft_with<N, BRn, TRn, BCn, TCn>(ctx, I, clip, nsl, floor_v);
   }
-  return launch_fft_with<N, BR, 256, BC, 256>(ctx, I, nsl, floor_v);
+  return launch_fft_with<N, BR, 256, BC, 256>(ctx, I, clip, nsl, floor_v);
 }
 
-// The phase-rotated uv grids of all slices (ctx->grid, one N x N grid each).
-gvm_status model_grids(gvm_ctx* ctx, const float* I, int nsl) {
-  const float floor_v = (float)(-1.0 * (double)ctx->g.eta * (double)ctx->g.minpix);
-  switch (ctx->g.N) {
-    case 64: return launch_fft<64>(ctx, I, nsl, floor_v);
-    case 128: return launch_fft<128>(ctx, I, nsl, floor_v);
-    case 256: return launch_fft<256>(ctx, I, nsl, floor_v);
-    case 512: return launch_fft<512>(ctx, I, nsl, floor_v);
-    case 1024: return launch_fft<1024>(ctx, I, nsl, floor_v);
-    case 2048: return launch_fft<2048>(ctx, I, nsl, floor_v);
-    case 4096: return launch_fft<4096>(ctx, I, nsl, floor_v);
+// clip2IWNoise on the caller's image and the phase-rotated uv grids of all
+// slices (ctx->grid, one N x N grid each).
+gvm_status model_grids(gvm_ctx* ctx, float* I, int nsl, int flag_opt) {
+  const gvm_geometry& g = ctx->g;
+  const float floor_v = (float)(-1.0 * (double)g.eta * (double)g.minpix);
+  const ClipArgs clip{ctx->noise, ctx->cmax, g.noise_cut, g.threshold, g.eta > 0.0f ? 1 : 0,
+                      flag_opt, nsl};
+  switch (g.N) {
+    case 64: return launch_fft<64>(ctx, I, clip, nsl, floor_v);
+    case 128: return launch_fft<128>(ctx, I, clip, nsl, floor_v);
+    case 256: return launch_fft<256>(ctx, I, clip, nsl, floor_v);
+    case 512: return launch_fft<512>(ctx, I, clip, nsl, floor_v);
+    case 1024: return launch_fft<1024>(ctx, I, clip, nsl, floor_v);
+    case 2048: return launch_fft<2048>(ctx, I, clip, nsl, floor_v);
+    case 4096: return launch_fft<4096>(ctx, I, clip, nsl, floor_v);
   }
   set_error("model_grid: unsupported N");
   return GVM_ERR_ARG;
@@ -268,7 +297,7 @@ gvm_status model_grids(gvm_ctx* ctx, const float* I, int nsl) {
 // ---------------------------------------------------------------------------
 // vis_mod + residual + chi2Vector + reduction:
 // src/functions.cu:2557-2609, 2663-2671, 2867-2876.  Bytes per visibility:
-// cell 4 + frac 8 + Vo 8 + weight 4 read, Vr 8 + polar c 8 written.
+// cell 4 + frac 8 + Vo 8 + weight 4 read, Vr 8 written.
 // ---------------------------------------------------------------------------
 constexpr int kDegridThreads = 512;
 
@@ -283,7 +312,6 @@ __global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
   const float2* __restrict__ vo = sd.vo;
   const float* __restrict__ weight = sd.weight;
   float2* __restrict__ vr = sd.vr;
-  float2* __restrict__ cpol = sd.cpol;
   const long nvis = sd.nvis;
   double acc = 0.0;
   float cm = 0.0f;
@@ -334,12 +362,9 @@ __global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
       }
       const float2 r = make_float2(o[u].x - m.x, o[u].y - m.y);
       stg(vr + v, r);
-      const float a2 = r.x * r.x + r.y * r.y;
+      const float a2 = resid_abs2(r);
       acc += (double)(w[u] * a2);
-      const float amp = w[u] * sqrtf(a2);
-      cm = fmaxf(cm, amp);
-      // c = w conj(Vr) in polar form for the adjoint: (|c|, arg(c) in turns)
-      stg(cpol + v, make_float2(amp, -atan2f(r.y, r.x) * 0.15915494309189535f));
+      cm = fmaxf(cm, w[u] * sqrtf(a2));  // |c|, as polar_kernel computes it
     }
   }
   const double tot = block_sum(acc, sh);
@@ -392,12 +417,8 @@ gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normal
   const long n = (long)g.M * g.N;
   const int nsl = (int)ctx->slices.size();
   GVM_HIP_TRY(hipSetDevice(ctx->device));
-  const float floor_v = (float)(-1.0 * (double)g.eta * (double)g.minpix);
-  hipLaunchKernelGGL(clip_kernel, dim3(reduce_grid(n)), dim3(256), 0, ctx->stream, I, ctx->noise,
-                     n, g.noise_cut, floor_v, g.eta > 0.0f ? 1 : 0, g.threshold, flag_opt,
-                     ctx->cmax, nsl);
-  GVM_CHECK_LAUNCH();
-  gvm_status st = model_grids(ctx, I, nsl);
+  (void)n;
+  gvm_status st = model_grids(ctx, I, nsl, flag_opt);
   if (st) return st;
   // blocks per slice: one visibility per thread, at most kReduceBlocks
   long vmax = 0;

@@ -194,6 +194,13 @@ __device__ __forceinline__ void stg(float2* p, float2 v) {
   *(__attribute__((address_space(1))) gf2_t*)p = w;
 }
 
+// |r|^2 of a residual with each operation rounded (no multiply-add
+// contraction), so the kernels that recompute it get the same bits.
+__device__ __forceinline__ float resid_abs2(float2 r) {
+#pragma clang fp contract(off)
+  return r.x * r.x + r.y * r.y;
+}
+
 // Wave-level (64 lanes) and block-level sums in double.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

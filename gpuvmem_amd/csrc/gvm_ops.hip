@@ -92,6 +92,104 @@ __global__ __launch_bounds__(256) void prior_grad_kernel(int kind, const float* 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Several priors in one pass (the objective's terms after Chi2).  Each kernel
+// repeats, per term, the arithmetic of the one-term kernels above in the same
+// order, so values and gradients are bit-identical to the one-term path.
+// ---------------------------------------------------------------------------
+struct PriorTermsDev {
+  gvm_prior_term t[GVM_MAX_FUSED_PRIORS];
+  int n;
+};
+
+// Block partials of every term's value: row t of partials for term t (grid
+// and per-thread order of prior_value_kernel).
+__global__ __launch_bounds__(256) void priors_value_kernel(PriorTermsDev T,
+                                                           const float* __restrict__ I,
+                                                           const float* __restrict__ noise,
+                                                           float noise_cut, int N,
+                                                           double* __restrict__ partials) {
+  __shared__ double sh[4];
+  const long n = (long)N * N;
+  double acc[GVM_MAX_FUSED_PRIORS];
+#pragma unroll
+  for (int t = 0; t < GVM_MAX_FUSED_PRIORS; t++) acc[t] = 0.0;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x) {
+    if (!(noise[k] < noise_cut)) continue;
+    const int i = (int)(k / N), j = (int)(k % N);
+#pragma unroll
+    for (int t = 0; t < GVM_MAX_FUSED_PRIORS; t++)
+      if (t < T.n && T.t[t].active)
+        acc[t] += (double)prior_term(T.t[t].kind, I + n * T.t[t].index, N, i, j, T.t[t].prior,
+                                     T.t[t].eta);
+  }
+#pragma unroll
+  for (int t = 0; t < GVM_MAX_FUSED_PRIORS; t++) {
+    if (t >= T.n) break;
+    const double tot = block_sum(acc[t], sh);
+    if (threadIdx.x == 0) partials[(long)t * kReduceBlocks + blockIdx.x] = tot;
+    __syncthreads();  // sh is reused by the next term
+  }
+}
+
+// dphi and xi from the chi2 gradient and every prior term's gradient, per
+// element in the order of the one-term path: dphi = chi2 gradient (copied,
+// or added to the cleared dphi with one image), then dphi[image_to_add] +=
+// lambda * dterm for each term in turn (an inactive term adds the +0 of its
+// cleared buffer), then xi = dphi.
+__global__ __launch_bounds__(256) void dphi_assemble_kernel(
+    PriorTermsDev T, const float* __restrict__ I, const float* __restrict__ noise,
+    float noise_cut, int N, int image_count, const float* __restrict__ chi2g,
+    float* __restrict__ dphi, float* __restrict__ xi) {
+#pragma clang fp contract(off)
+  const long n = (long)N * N;
+  const long total = n * image_count;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < total;
+       k += (long)gridDim.x * blockDim.x) {
+    const int img = (int)(k / n);
+    const long kk = k - (long)img * n;
+    float r = 0.0f;
+    if (chi2g) r = image_count == 1 ? 0.0f + chi2g[k] : chi2g[k];
+    const bool inside = noise[kk] < noise_cut;
+#pragma unroll
+    for (int t = 0; t < GVM_MAX_FUSED_PRIORS; t++) {
+      if (t >= T.n) break;
+      if (T.t[t].image_to_add != img) continue;
+      float ds = 0.0f;
+      if (T.t[t].active) {
+        float d = 0.0f;
+        if (inside)
+          d = prior_grad(T.t[t].kind, I + n * T.t[t].index, N, (int)(kk / N), (int)(kk % N),
+                         T.t[t].prior, T.t[t].eta);
+        ds = d * T.t[t].lambda;
+      }
+      r = r + ds;
+    }
+    dphi[k] = r;
+    xi[k] = r;
+  }
+}
+
+static gvm_status check_terms(const gvm_ctx* ctx, const gvm_prior_term* terms, int n,
+                              PriorTermsDev* out) {
+  GVM_REQUIRE(n >= 0 && n <= GVM_MAX_FUSED_PRIORS,
+              "fused priors: at most GVM_MAX_FUSED_PRIORS terms");
+  GVM_REQUIRE(n == 0 || terms, "fused priors: null term list");
+  *out = PriorTermsDev{};
+  out->n = n;
+  for (int t = 0; t < n; t++) {
+    const gvm_prior_term& p = terms[t];
+    GVM_REQUIRE(p.kind >= GVM_PRIOR_ENTROPY && p.kind <= GVM_PRIOR_LAPLACIAN,
+                "fused priors: unknown kind");
+    GVM_REQUIRE(p.index >= 0 && p.index < ctx->g.image_count &&
+                    p.image_to_add >= 0 && p.image_to_add < ctx->g.image_count,
+                "fused priors: bad image index");
+    out->t[t] = p;
+  }
+  return GVM_OK;
+}
+
 static gvm_status prior_value_enqueue(gvm_ctx* ctx, int kind, const float* I, int index,
                                       float G, float eta, int slot) {
   GVM_REQUIRE(ctx && I, "prior value: null argument");
@@ -328,6 +426,42 @@ extern "C" gvm_status gvm_laplacian_value(gvm_ctx* ctx, const float* I, int inde
 extern "C" gvm_status gvm_laplacian_grad(gvm_ctx* ctx, const float* I, float* dgi, int index,
                                          float lambda) {
   return prior_grad(ctx, PRIOR_LAPLACIAN, I, dgi, index, 0.f, 0.f, lambda);
+}
+
+extern "C" gvm_status gvm_priors_value_enqueue(gvm_ctx* ctx, const float* I,
+                                               const gvm_prior_term* terms, int n,
+                                               int first_slot) {
+  GVM_REQUIRE(ctx && I, "gvm_priors_value_enqueue: null argument");
+  PriorTermsDev T;
+  gvm_status st = check_terms(ctx, terms, n, &T);
+  if (st) return st;
+  GVM_REQUIRE(first_slot >= 0 && first_slot + n <= kValueSlots,
+              "gvm_priors_value_enqueue: value slots out of range");
+  if (n == 0) return GVM_OK;
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  const int N = ctx->g.N;
+  const long np = (long)N * N;
+  const int nb = reduce_grid(np);
+  hipLaunchKernelGGL(priors_value_kernel, dim3(nb), dim3(256), 0, ctx->stream, T, I, ctx->noise,
+                     ctx->g.noise_cut, N, ctx->partials);
+  GVM_CHECK_LAUNCH();
+  // one block per term: finish_kernel's order, into consecutive value slots
+  return finish_reductions_to(ctx, n, nb, ctx->dvalues + first_slot);
+}
+
+extern "C" gvm_status gvm_dphi_assemble(gvm_ctx* ctx, const float* I, const float* chi2_grad,
+                                        float* dphi, float* xi, const gvm_prior_term* terms,
+                                        int n) {
+  GVM_REQUIRE(ctx && I && dphi && xi, "gvm_dphi_assemble: null argument");
+  PriorTermsDev T;
+  gvm_status st = check_terms(ctx, terms, n, &T);
+  if (st) return st;
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  const int N = ctx->g.N;
+  const long total = (long)N * N * ctx->g.image_count;
+  LAUNCH_1D(dphi_assemble_kernel, total, T, I, ctx->noise, ctx->g.noise_cut, N,
+            ctx->g.image_count, chi2_grad, dphi, xi);
+  return GVM_OK;
 }
 
 extern "C" gvm_status gvm_add_to_dphi(gvm_ctx* ctx, float* dphi, const float* dgi, int index) {

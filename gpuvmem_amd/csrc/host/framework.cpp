@@ -128,7 +128,7 @@ std::vector<int64_t> ObjectiveFunction::baseKey() const {
   check(gvm_epoch(gvm::ctx(), &epoch), "gvm_epoch");
   const auto& g = globals();
   return {key_addr(gvm::ctx()), epoch, g.flag_opt, g.M, g.N, g.image_count, key_addr(dphi),
-          (int64_t)fis.size()};
+          (int64_t)fis.size(), fusable() ? 1 : 0};
 }
 
 // The cache entry of `key` (created if new; the least recently used entry is
@@ -162,6 +162,33 @@ void ObjectiveFunction::addFi(Fi* fi) {
   }
 }
 
+// Chi2 first, then priors that have a one-pass form.
+bool ObjectiveFunction::fusable() const {
+  if (!fused || fis.empty() || fis.size() - 1 > GVM_MAX_FUSED_PRIORS) return false;
+  if (!dynamic_cast<Chi2*>(fis[0])) return false;
+  gvm_prior_term t;
+  for (size_t k = 1; k < fis.size(); k++)
+    if (!fis[k]->priorTerm(&t, false)) return false;
+  return true;
+}
+
+// Enqueues every term's value into slot k; queued[k] says whether term k
+// produced one (an inactive prior is 0 without evaluation).
+void ObjectiveFunction::enqueueValues(float* p, std::vector<char>& queued) {
+  if (!fusable()) {
+    for (size_t k = 0; k < fis.size(); k++) queued[k] = fis[k]->enqueueValue(p, (int)k) ? 1 : 0;
+    return;
+  }
+  queued[0] = fis[0]->enqueueValue(p, 0) ? 1 : 0;
+  std::vector<gvm_prior_term> terms(fis.size() - 1);
+  for (size_t k = 1; k < fis.size(); k++) {
+    fis[k]->priorTerm(&terms[k - 1], false);
+    queued[k] = terms[k - 1].active ? 1 : 0;
+  }
+  check(gvm_priors_value_enqueue(gvm::ctx(), p, terms.data(), (int)terms.size(), 1),
+        "gvm_priors_value_enqueue");
+}
+
 double ObjectiveFunction::calcFunctionD(float* p) {
   double value = 0.0;
   n_fcalls++;
@@ -183,7 +210,7 @@ double ObjectiveFunction::calcFunctionD(float* p) {
         check(gvm_capture_begin(gvm::ctx()), "gvm_capture_begin");
         bool ok = true;
         try {
-          for (size_t k = 0; k < fis.size(); k++) queued[k] = fis[k]->enqueueValue(p, (int)k);
+          enqueueValues(p, queued);
           check(gvm_values_fetch_enqueue(gvm::ctx(), nv), "gvm_values_fetch_enqueue");
         } catch (const gvm::Error&) {
           ok = false;
@@ -204,7 +231,7 @@ double ObjectiveFunction::calcFunctionD(float* p) {
       queued = e->queued;
       graph_launches++;
     } else {
-      for (size_t k = 0; k < fis.size(); k++) queued[k] = fis[k]->enqueueValue(p, (int)k) ? 1 : 0;
+      enqueueValues(p, queued);
       check(gvm_values_read(gvm::ctx(), nv, raw.data()), "gvm_values_read");
       if (e) e->seen++;
     }
@@ -224,6 +251,20 @@ double ObjectiveFunction::calcFunctionD(float* p) {
 float ObjectiveFunction::calcFunction(float* p) { return (float)calcFunctionD(p); }
 
 void ObjectiveFunction::gradientSequence(float* p, float* xi, int iter) {
+  if (fusable()) {
+    // Chi2's restart and gradient, then the priors' gradients, the
+    // AddToDPhi sequence and the copy to xi in one kernel
+    for (Fi* fi : fis) fi->setIteration(iter);
+    Chi2* chi2 = static_cast<Chi2*>(fis[0]);
+    chi2->restartDGi();
+    chi2->calcGi(p, xi);
+    std::vector<gvm_prior_term> terms(fis.size() - 1);
+    for (size_t k = 1; k < fis.size(); k++) fis[k]->priorTerm(&terms[k - 1], true);
+    check(gvm_dphi_assemble(gvm::ctx(), p, chi2->getResult(), dphi, xi, terms.data(),
+                            (int)terms.size()),
+          "gvm_dphi_assemble");
+    return;
+  }
   restartDPhi();
   for (Fi* fi : fis) {
     fi->setIteration(iter);
@@ -369,6 +410,38 @@ void Chi2::addToDphi(float* device_dphi) {
 // flag_opt % 2 == imageIndex.
 // ---------------------------------------------------------------------------
 static bool prior_active(int iteration, float lambda) { return iteration > 0 && lambda != 0.0f; }
+
+// The one-pass descriptor of a prior (value gating; for a gradient also the
+// flag_opt gating of calcGi).
+static void fill_prior_term(gvm_prior_term* t, int kind, int imageIndex, int imageToAdd,
+                            float prior, float eta, float lambda, int iteration, bool gradient) {
+  t->kind = kind;
+  t->index = imageIndex;
+  t->image_to_add = imageToAdd;
+  t->prior = prior;
+  t->eta = eta;
+  t->lambda = lambda;
+  t->pad = 0.0f;
+  bool on = prior_active(iteration, lambda);
+  if (gradient) on = on && globals().flag_opt % 2 == imageIndex;
+  t->active = on ? 1 : 0;
+}
+
+bool Entropy::priorTerm(gvm_prior_term* t, bool gradient) const {
+  fill_prior_term(t, GVM_PRIOR_ENTROPY, imageIndex, imageToAdd, prior_value, eta,
+                  penalization_factor, iteration, gradient);
+  return true;
+}
+bool TotalSquaredVariationP::priorTerm(gvm_prior_term* t, bool gradient) const {
+  fill_prior_term(t, GVM_PRIOR_TSV, imageIndex, imageToAdd, 0.f, 0.f, penalization_factor,
+                  iteration, gradient);
+  return true;
+}
+bool Laplacian::priorTerm(gvm_prior_term* t, bool gradient) const {
+  fill_prior_term(t, GVM_PRIOR_LAPLACIAN, imageIndex, imageToAdd, 0.f, 0.f, penalization_factor,
+                  iteration, gradient);
+  return true;
+}
 
 Entropy::Entropy() : prior_value(1.0f), eta(-1.0f) { this->name = "Entropy"; }
 Entropy::Entropy(float prior_value) : prior_value(prior_value), eta(-1.0f) { this->name = "Entropy"; }

@@ -204,6 +204,15 @@ extern "C" gvm_status gvm_fw_set_graphs(gvm_fw_objective* obj, int on) {
   return GVM_OK;
 }
 
+extern "C" gvm_status gvm_fw_set_fused_terms(gvm_fw_objective* obj, int on) {
+  if (!obj || !obj->of) {
+    g_fw_error = "gvm_fw_set_fused_terms: bad argument";
+    return GVM_ERR_ARG;
+  }
+  obj->of->setFusedTerms(on != 0);
+  return GVM_OK;
+}
+
 extern "C" gvm_status gvm_fw_set_lbfgs_host_scalars(gvm_fw_objective* obj, int on) {
   if (!obj) {
     g_fw_error = "gvm_fw_set_lbfgs_host_scalars: null objective";

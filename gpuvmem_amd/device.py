@@ -314,6 +314,10 @@ class Objective:
         """Replay captured HIP graphs of repeated evaluations."""
         native.check_fw(self.fw.gvm_fw_set_graphs(self.h, int(on)), "gvm_fw_set_graphs")
 
+    def set_fused_terms(self, on: bool) -> None:
+        """Priors after Chi2 in one pass (default) or term by term."""
+        native.check_fw(self.fw.gvm_fw_set_fused_terms(self.h, int(on)), "gvm_fw_set_fused_terms")
+
     def set_lbfgs_host_scalars(self, on: bool) -> None:
         """CG-LBFGS reads every dot product of its recursion back to the host
         (the reference's way) instead of keeping the scalars on the device."""

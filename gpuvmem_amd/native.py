@@ -179,6 +179,8 @@ HIP_API = {
     "gvm_laplacian_value": [_P, _P, _I, _DP],
     "gvm_laplacian_grad": [_P, _P, _P, _I, _F],
     "gvm_add_to_dphi": [_P, _P, _P, _I],
+    "gvm_priors_value_enqueue": [_P, _P, _P, _I, _I],
+    "gvm_dphi_assemble": [_P, _P, _P, _P, _P, _P, _I],
     "gvm_vec_zero": [_P, _P, _L],
     "gvm_vec_copy": [_P, _P, _P, _L],
     "gvm_cg_search_direction": [_P, _P, _P, _P, _I],
@@ -239,6 +241,7 @@ FW_API = {
     "gvm_fw_set_flag": ([_P, _I], _I),
     "gvm_fw_set_graphs": ([_P, _I], _I),
     "gvm_fw_set_lbfgs_host_scalars": ([_P, _I], _I),
+    "gvm_fw_set_fused_terms": ([_P, _I], _I),
     "gvm_fw_set_line_search_replay": ([_P, C.POINTER(C.c_double), _I], _I),
     "gvm_fw_graph_launches": ([_P, C.POINTER(_L)], _I),
     "gvm_fw_last_error": ([], C.c_char_p),

@@ -42,6 +42,11 @@ class Fi {
     set_fivalue(raw);
     return (double)penalization_factor * raw;
   }
+  // A prior evaluated with others in one pass (not in the reference): fills
+  // t with this term's kind, images, constants and activity (the value's
+  // gating, and for `gradient` also the flag_opt gating of calcGi) and
+  // returns true; false for a term that has no such form.
+  virtual bool priorTerm(gvm_prior_term* t, bool gradient) const { return false; }
   virtual void setPrior(float prior) {}
   virtual void setPrior(float* prior) {}
   virtual float getEta() { return 0.0f; }

@@ -89,6 +89,9 @@ gvm_status gvm_fw_graph_launches(gvm_fw_objective* obj, long* n);
  * scalars stay on the device and the recursion runs without host round
  * trips.  Both give the same search direction, bit for bit. */
 gvm_status gvm_fw_set_lbfgs_host_scalars(gvm_fw_objective* obj, int on);
+/* ObjectiveFunction::setFusedTerms: the priors after Chi2 in one pass (on by
+ * default; off gives the term-by-term launches, with identical results). */
+gvm_status gvm_fw_set_fused_terms(gvm_fw_objective* obj, int on);
 /* Line-search replay (no counterpart in the reference): the next
  * gvm_fw_optimize takes xmins[0..n) in turn as linmin's step lengths instead of
  * bracketing and minimising (src/linmin.cu:52-110), and fails with

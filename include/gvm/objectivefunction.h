@@ -35,6 +35,13 @@ class ObjectiveFunction {
   void setGraphs(bool on);
   bool getGraphs() const { return graphs; }
   long getGraphLaunches() const { return graph_launches; }
+  // With Chi2 first and up to GVM_MAX_FUSED_PRIORS priors after it, their
+  // values are enqueued in one pass and the gradient's restart / AddToDPhi /
+  // copy sequence is one kernel (gvm_priors_value_enqueue,
+  // gvm_dphi_assemble), bit-identical to the term-by-term sequence.  On by
+  // default.
+  void setFusedTerms(bool on) { fused = on; }
+  bool getFusedTerms() const { return fused; }
 
  private:
   struct GraphEntry {
@@ -49,6 +56,9 @@ class ObjectiveFunction {
   std::vector<int64_t> baseKey() const;
   void clearGraphs();
   void gradientSequence(float* p, float* xi, int iter);
+  bool fusable() const;
+  void enqueueValues(float* p, std::vector<char>& queued);
+  bool fused = true;
   bool graphs = false;
   std::vector<GraphEntry> value_graphs, grad_graphs;
   long graph_clock = 0, graph_launches = 0;

@@ -21,6 +21,8 @@ class Chi2 : public Fi {
   void configure(int penalizatorIndex, int imageIndex, int imageToAdd, bool normalize) override;
   void setFgScale(float fg_scale) override { this->fg_scale = fg_scale; }
   float getFgScale() override { return this->fg_scale; }
+  // the device gradient buffer calcGi accumulates into
+  float* getResult() const { return result_dchi2; }
 
  private:
   float* result_dchi2 = nullptr;
@@ -40,6 +42,7 @@ class Entropy : public Fi {
   void calcGi(float* p, float* xi) override;
   void restartDGi() override;
   void addToDphi(float* device_dphi) override;
+  bool priorTerm(gvm_prior_term* t, bool gradient) const override;
   float getPrior() { return prior_value; }
   void setPrior(float prior_value) override { this->prior_value = prior_value; }
   float getEta() override { return eta; }
@@ -53,6 +56,7 @@ class Entropy : public Fi {
 class TotalSquaredVariationP : public Fi {
  public:
   TotalSquaredVariationP();
+  bool priorTerm(gvm_prior_term* t, bool gradient) const override;
   float calcFi(float* p) override;
   double calcFiD(float* p) override;
   bool hasAsyncValue() const override { return true; }
@@ -65,6 +69,7 @@ class TotalSquaredVariationP : public Fi {
 class Laplacian : public Fi {
  public:
   Laplacian();
+  bool priorTerm(gvm_prior_term* t, bool gradient) const override;
   float calcFi(float* p) override;
   double calcFiD(float* p) override;
   bool hasAsyncValue() const override { return true; }

@@ -312,6 +312,41 @@ gvm_status gvm_laplacian_value(gvm_ctx* ctx, const float* I_dev, int index,
 gvm_status gvm_laplacian_grad(gvm_ctx* ctx, const float* I_dev,
                               float* dgi_dev, int index, float lambda);
 
+/* ---- several priors in one pass ------------------------------------------ */
+/* No counterpart in the reference, which launches each term's value, each
+ * term's gradient and each AddToDPhi separately (src/functions.cu:4722-4964,
+ * 4560-4564).  A term: its kind, the image it reads, the image of dphi its
+ * gradient adds to, its value slot's order, entropy's prior G and eta
+ * (ignored by the others), the gradient factor lambda, and whether it is
+ * active (the reference's iter > 0 and flag_opt gating, decided by the
+ * caller; an inactive value is 0 and an inactive gradient adds +0).  Results
+ * are bit-identical to the one-term calls in the same order. */
+#define GVM_PRIOR_ENTROPY 0
+#define GVM_PRIOR_TSV 1
+#define GVM_PRIOR_LAPLACIAN 2
+#define GVM_MAX_FUSED_PRIORS 4
+typedef struct {
+  int kind;
+  int index;
+  int image_to_add;
+  int active;
+  float prior;
+  float eta;
+  float lambda;
+  float pad;
+} gvm_prior_term;
+/* The values of n <= GVM_MAX_FUSED_PRIORS terms into value slots
+ * [first_slot, first_slot + n) (read with gvm_values_read / values_host);
+ * one pass over the image for all terms. */
+gvm_status gvm_priors_value_enqueue(gvm_ctx* ctx, const float* I_dev, const gvm_prior_term* terms,
+                                    int n, int first_slot);
+/* ObjectiveFunction's gradient assembly after Chi2::calcGi in one pass:
+ * dphi = chi2_grad (NULL: zero), then dphi[term.image_to_add] += lambda *
+ * dterm for each term in order, then xi = dphi; every buffer
+ * float[image_count*M*N]. */
+gvm_status gvm_dphi_assemble(gvm_ctx* ctx, const float* I_dev, const float* chi2_grad_dev,
+                             float* dphi_dev, float* xi_dev, const gvm_prior_term* terms, int n);
+
 /* linkAddToDPhi -> AddToDPhi (src/functions.cu:4560-4564, 3890-3895):
  * dphi[index] += dgi. */
 gvm_status gvm_add_to_dphi(gvm_ctx* ctx, float* dphi_dev, const float* dgi_dev,

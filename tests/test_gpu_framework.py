@@ -299,3 +299,55 @@ def test_unknown_optimizer_is_an_error():
         Objective(ctx, fw_config([("NoSuchTerm", -1, 0.0)]))
     obj.close()
     ctx.close()
+
+
+def _terms_cfg(specs, lambdas, flag_opt=0):
+    """specs: (name, penalizator_index, image_index, image_to_add, prior)."""
+    cfg = fw_config([(n, pen, prior) for n, pen, _, _, prior in specs], lambdas=lambdas,
+                    flag_opt=flag_opt)
+    for k, (_, pen, idx, add, prior) in enumerate(specs):
+        cfg.terms[k].image_index = idx
+        cfg.terms[k].image_to_add = add
+    return cfg
+
+
+@pytest.mark.parametrize("flag_opt", [0, 1])
+def test_fused_prior_terms_are_bitwise_the_term_by_term_sequence(flag_opt):
+    """Values and gradients of Chi2 + Entropy + TSV + Laplacian (priors on both
+    images, one adding to the other image) with the priors in one pass equal
+    the term-by-term launches bit for bit, at iteration 0 (priors gated off)
+    and iteration 1, and so do five CG iterations."""
+    p = small_problem(npix=64, n_antennas=8, seed=12)
+    print(f"case: fused priors flag_opt={flag_opt} {p.M}^2 nvis={p.nvis}", flush=True)
+    specs = [("Chi2", -1, 0, 0, 0.0), ("Entropy", 0, 0, 0, 0.001),
+             ("TotalSquaredVariation", 1, 0, 0, 0.0), ("Laplacian", 2, 1, 1, 0.0),
+             ("TotalSquaredVariation", 1, 1, 0, 0.0)]
+    img = p.initial_image()
+    rng = np.random.default_rng(3)
+    img[0] += (rng.random((p.N, p.M)) * 20).astype(np.float32)
+    img[1] = (rng.standard_normal((p.N, p.M)) * 0.3).astype(np.float32)
+    out = {}
+    for fused in (True, False):
+        ctx = Context(p)
+        obj = Objective(ctx, _terms_cfg(specs, (0.01, 1e-3, 2e-4), flag_opt=flag_opt))
+        obj.set_fused_terms(fused)
+        d = ctx.image(img)
+        xi = ctx.zeros(2 * p.M * p.N)
+        rec = []
+        for it in (0, 1, 1):
+            v = obj.value(d)
+            obj.gradient(d, xi, it)
+            rec.append((v, obj.term_values(), xi.download(np.float32).copy()))
+        e = ctx.image(p.initial_image())
+        res, traj = obj.optimize(e, "CG-FRPRMN", 5)
+        rec.append((traj, e.download(np.float32).copy()))
+        out[fused] = rec
+        obj.close()
+        ctx.close()
+    for a, b in zip(out[True][:3], out[False][:3]):
+        assert a[0] == b[0] and a[1] == b[1]
+        assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
+    assert out[True][3][0] == out[False][3][0]
+    assert np.array_equal(out[True][3][1], out[False][3][1])
+    # the priors did act at iteration 1
+    assert any(v != 0.0 for v in out[True][2][1][1:])
