@@ -722,6 +722,7 @@ def main() -> int:
             "algorithmic_flop_per_launch": flops,
         },
     }
+    line["value_evaluation_us"] = value_latency(obj, d_img, ctx)
     if args.calibrate_copy_mib > 0:
         copy_calibration(ctx, args.calibrate_copy_mib)
     obj.close()
@@ -736,6 +737,25 @@ def main() -> int:
         line["cpu_baseline"] = cpu_baseline(p, host_cpus(), args.cpu_seconds)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def value_latency(obj, d_img, ctx, n: int = 200) -> dict:
+    """Wall time of one value-only evaluation (ObjectiveFunction::calcFunction,
+    what each line-search probe costs: forward chi2 and prior values, read
+    back to the host), with evaluation graphs replayed and without."""
+    out = {}
+    for graphs in (True, False):
+        obj.set_graphs(graphs)
+        for _ in range(3):
+            obj.value(d_img)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            obj.value(d_img)
+        out["graphs" if graphs else "eager"] = (time.perf_counter() - t0) / n * 1e6
+    obj.set_graphs(False)
+    out["evaluations"] = n
+    return out
 
 
 def plain_fp32_step(p, steps: int, warmup: int, graphs: bool) -> dict:

@@ -570,6 +570,19 @@ extern "C" gvm_status gvm_chi2_enqueue(gvm_ctx* ctx, float* I_dev, int flag_opt,
   return forward_chi2_enqueue(ctx, I_dev, flag_opt, normalize, slot);
 }
 
+extern "C" gvm_status gvm_values_enqueue(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize,
+                                         int chi2_slot, const gvm_prior_term* terms, int n,
+                                         int first_slot) {
+  GVM_REQUIRE(ctx && I_dev, "gvm_values_enqueue: null argument");
+  GVM_REQUIRE(!ctx->slices.empty(), "gvm_values_enqueue: no visibilities");
+  GVM_REQUIRE(flag_opt >= 0, "gvm_values_enqueue: negative flag_opt");
+  GVM_REQUIRE(chi2_slot >= 0 && chi2_slot < kValueSlots && first_slot >= 0 && n >= 0 &&
+                  first_slot + n <= kValueSlots &&
+                  (n == 0 || chi2_slot < first_slot || chi2_slot >= first_slot + n),
+              "gvm_values_enqueue: value slots out of range or overlapping");
+  return values_enqueue(ctx, I_dev, flag_opt, normalize, chi2_slot, terms, n, first_slot);
+}
+
 extern "C" gvm_status gvm_values_read(gvm_ctx* ctx, int n, double* out) {
   GVM_REQUIRE(ctx && out, "gvm_values_read: null argument");
   GVM_REQUIRE(n >= 1 && n <= kValueSlots, "gvm_values_read: slot count out of range");

@@ -86,18 +86,35 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 //   rows: element (t, p) at t * N + p;  cols: element (t, p) at p * B + t.
 // All reads of the stage complete (barrier) before any write, so the stage
 // runs in place in one buffer.
+// Radix-4 butterfly in place: b_m = sum_r a_r e^{+2 pi i r m / 4}.
+__device__ __forceinline__ void bfly4(float2& a0, float2& a1, float2& a2, float2& a3) {
+#pragma clang fp contract(off)
+  const float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
+  const float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
+  const float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
+  const float2 d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
+  a0 = make_float2(s02.x + s13.x, s02.y + s13.y);
+  a2 = make_float2(s02.x - s13.x, s02.y - s13.y);
+  // i * d13 = (-d13.y, d13.x)
+  a1 = make_float2(d02.x - d13.y, d02.y + d13.x);
+  a3 = make_float2(d02.x + d13.y, d02.y - d13.x);
+}
+
+// One Stockham stage (radix R = 2, 4 or 8, current sub-transform length NS)
+// over B transforms of length N held in LDS.  A thread takes butterflies
+// q = threadIdx.x + b T while q < B N / R.
 template <int N, int B, bool COLS, int R, int NS, int T>
 __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __restrict__ tw) {
 #pragma clang fp contract(off)
   constexpr int NB = B * N / R;          // butterflies in this stage
-  constexpr int BPT = NB / T;            // per thread
-  static_assert(NB % T == 0, "butterflies must divide evenly over the threads");
+  constexpr int BPT = (NB + T - 1) / T;  // per thread (the last round may be partial)
   constexpr int STRIDE = N / R;
   float2 v[BPT][R];
   int base_t[BPT], out_j[BPT];
 #pragma unroll
   for (int b = 0; b < BPT; b++) {
     const int q = threadIdx.x + b * T;
+    if (NB % T != 0 && q >= NB) break;
     const int t = COLS ? (q % B) : (q / STRIDE);
     const int j = COLS ? (q / B) : (q % STRIDE);
     const int k = j % NS;
@@ -113,18 +130,28 @@ __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __rest
         v[b][r] = cmul(v[b][r], tw[e]);
       }
     }
-    if (R == 4) {
-      // b_m = sum_r a_r e^{+2 pi i r m / 4}
-      const float2 a0 = v[b][0], a1 = v[b][1], a2 = v[b][2], a3 = v[b][3];
-      const float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
-      const float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
-      const float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
-      const float2 d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
-      v[b][0] = make_float2(s02.x + s13.x, s02.y + s13.y);
-      v[b][2] = make_float2(s02.x - s13.x, s02.y - s13.y);
-      // i * d13 = (-d13.y, d13.x)
-      v[b][1] = make_float2(d02.x - d13.y, d02.y + d13.x);
-      v[b][3] = make_float2(d02.x + d13.y, d02.y - d13.x);
+    if constexpr (R == 8) {
+      // b_m = sum_r a_r e^{+2 pi i r m / 8}: radix-4 over the even and the
+      // odd inputs, then b_m = E_m + W^m O_m, b_{m+4} = E_m - W^m O_m with
+      // W = e^{+2 pi i / 8}
+      float2 e0 = v[b][0], e1 = v[b][2], e2 = v[b][4], e3 = v[b][6];
+      float2 o0 = v[b][1], o1 = v[b][3], o2 = v[b][5], o3 = v[b][7];
+      bfly4(e0, e1, e2, e3);
+      bfly4(o0, o1, o2, o3);
+      constexpr float h = 0.70710678118654752f;  // sqrt(1/2)
+      const float2 w1 = make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y));     // W o1
+      const float2 w2 = make_float2(-o2.y, o2.x);                              // i o2
+      const float2 w3 = make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y));    // W^3 o3
+      v[b][0] = make_float2(e0.x + o0.x, e0.y + o0.y);
+      v[b][4] = make_float2(e0.x - o0.x, e0.y - o0.y);
+      v[b][1] = make_float2(e1.x + w1.x, e1.y + w1.y);
+      v[b][5] = make_float2(e1.x - w1.x, e1.y - w1.y);
+      v[b][2] = make_float2(e2.x + w2.x, e2.y + w2.y);
+      v[b][6] = make_float2(e2.x - w2.x, e2.y - w2.y);
+      v[b][3] = make_float2(e3.x + w3.x, e3.y + w3.y);
+      v[b][7] = make_float2(e3.x - w3.x, e3.y - w3.y);
+    } else if constexpr (R == 4) {
+      bfly4(v[b][0], v[b][1], v[b][2], v[b][3]);
     } else {
       const float2 a0 = v[b][0], a1 = v[b][1];
       v[b][0] = make_float2(a0.x + a1.x, a0.y + a1.y);
@@ -136,6 +163,7 @@ __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __rest
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < BPT; b++) {
+    if (NB % T != 0 && threadIdx.x + b * T >= NB) break;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int p = out_j[b] + r * NS;
@@ -148,33 +176,41 @@ __device__ __forceinline__ void stockham_stage(float2* lds, const float2* __rest
   __syncthreads();
 }
 
-template <int N, int B, bool COLS, int NS, int T>
-__device__ __forceinline__ void radix4_stages(float2* lds, const float2* __restrict__ tw) {
-  if constexpr (NS < N) {
-    stockham_stage<N, B, COLS, 4, NS, T>(lds, tw);
-    radix4_stages<N, B, COLS, NS * 4, T>(lds, tw);
-  }
-}
-
 template <int N>
-constexpr bool log2_is_odd() {
+constexpr int log2_of() {
   int l = 0;
   int n = N;
   while (n > 1) {
     n >>= 1;
     l++;
   }
-  return (l & 1) != 0;
+  return l;
 }
 
+// Radix of the stage that starts at sub-transform length NS: radix-8 stages
+// first, then radix-4 (two when log2 N = 1 mod 3, one when 2 mod 3).
+template <int N, int NS>
+constexpr int stage_radix() {
+  constexpr int left = log2_of<N>() - log2_of<NS>();  // log2 of what remains
+  if (left % 3 == 0) return 8;
+  if (left % 3 == 1) return left >= 4 ? (left == 4 ? 4 : 8) : 2;
+  return left == 2 ? 4 : 8;
+}
+
+template <int N, int B, bool COLS, int NS, int T>
+__device__ __forceinline__ void fft_stages(float2* lds, const float2* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int R = stage_radix<N, NS>();
+    stockham_stage<N, B, COLS, R, NS, T>(lds, tw);
+    fft_stages<N, B, COLS, NS * R, T>(lds, tw);
+  }
+}
+
+// N-point transforms (e^{+2 pi i}, unnormalised) of the B sequences in LDS:
+// N = 8^a 4^b with b <= 2 (2 for N = 2), e.g. 1024 = 8 8 4 4.
 template <int N, int B, bool COLS, int T>
 __device__ __forceinline__ void fft_lds(float2* lds, const float2* __restrict__ tw) {
-  if constexpr (log2_is_odd<N>()) {
-    stockham_stage<N, B, COLS, 2, 1, T>(lds, tw);
-    radix4_stages<N, B, COLS, 2, T>(lds, tw);
-  } else {
-    radix4_stages<N, B, COLS, 1, T>(lds, tw);
-  }
+  fft_stages<N, B, COLS, 1, T>(lds, tw);
 }
 
 // Row pass: clip2IWNoise on the rows' pixels (written back by the slice-0
@@ -412,12 +448,41 @@ __global__ void chi2_combine_kernel(const double* __restrict__ sums,
   *out = 0.5 * total;
 }
 
-gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot) {
+// chi2 (block 0, chi2_finish_kernel's arithmetic) and prior values (block
+// 1 + t: finish_kernel's arithmetic on partial row row0 + t into out[slot0 +
+// t]) in one launch, bit-identical to the separate finishing launches.
+__global__ __launch_bounds__(kReduceThreads) void values_finish_kernel(
+    const double* __restrict__ partials, int nblocks_chi2, const SliceDev* __restrict__ tab,
+    int nsl, int normalize, double* __restrict__ chi2_out, int row0, int nblocks_prior,
+    double* __restrict__ prior_out) {
+  __shared__ double sh[kReduceThreads / 64];
+  if (blockIdx.x == 0) {
+    double total = 0.0;
+    for (int s = 0; s < nsl; s++) {
+      const double* p = partials + (long)s * kReduceBlocks;
+      double v = 0.0;
+      for (int i = threadIdx.x; i < nblocks_chi2; i += blockDim.x) v = v + p[i];
+      const double r = block_sum(v, sh);
+      if (threadIdx.x == 0) total += normalize ? r / (double)tab[s].nvis : r;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) *chi2_out = 0.5 * total;
+    return;
+  }
+  const int t = blockIdx.x - 1;
+  const double* p = partials + (long)(row0 + t) * kReduceBlocks;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nblocks_prior; i += blockDim.x) v = v + p[i];
+  const double r = block_sum(v, sh);
+  if (threadIdx.x == 0) prior_out[t] = r;
+}
+
+// The forward pass up to the chi2 block partials (slice s in partial row s);
+// returns the blocks per slice.
+static gvm_status forward_chi2_partials(gvm_ctx* ctx, float* I, int flag_opt, int* nb_out) {
   const gvm_geometry& g = ctx->g;
-  const long n = (long)g.M * g.N;
   const int nsl = (int)ctx->slices.size();
   GVM_HIP_TRY(hipSetDevice(ctx->device));
-  (void)n;
   gvm_status st = model_grids(ctx, I, nsl, flag_opt);
   if (st) return st;
   // blocks per slice: one visibility per thread, at most kReduceBlocks
@@ -426,9 +491,39 @@ gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normal
   const int nb = (int)std::min<long>(
       kReduceBlocks, std::max<long>(1, (vmax + kDegridThreads - 1) / kDegridThreads));
   hipLaunchKernelGGL(degrid_chi2_kernel, dim3(nb, nsl), dim3(kDegridThreads), 0, ctx->stream,
-                     ctx->grid,
-                     g.N, ctx->slice_tab, ctx->partials, ctx->cmax);
+                     ctx->grid, g.N, ctx->slice_tab, ctx->partials, ctx->cmax);
   GVM_CHECK_LAUNCH();
+  *nb_out = nb;
+  return GVM_OK;
+}
+
+gvm_status values_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int chi2_slot,
+                          const gvm_prior_term* terms, int n, int first_slot) {
+  const int nsl = (int)ctx->slices.size();
+  if (nsl > 4) {
+    // many slices: the chi2 finish takes its own launches
+    gvm_status st = forward_chi2_enqueue(ctx, I, flag_opt, normalize, chi2_slot);
+    if (st) return st;
+    return priors_value_partials_finish(ctx, I, terms, n, first_slot);
+  }
+  int nb = 0;
+  gvm_status st = forward_chi2_partials(ctx, I, flag_opt, &nb);
+  if (st) return st;
+  int nbp = 0;
+  if ((st = priors_value_partials(ctx, I, terms, n, nsl, &nbp))) return st;
+  hipLaunchKernelGGL(values_finish_kernel, dim3(1 + n), dim3(kReduceThreads), 0, ctx->stream,
+                     ctx->partials, nb, ctx->slice_tab, nsl, normalize, ctx->dvalues + chi2_slot,
+                     nsl, nbp, ctx->dvalues + first_slot);
+  GVM_CHECK_LAUNCH();
+  ctx->have_residuals = true;
+  return GVM_OK;
+}
+
+gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot) {
+  const int nsl = (int)ctx->slices.size();
+  int nb = 0;
+  gvm_status st = forward_chi2_partials(ctx, I, flag_opt, &nb);
+  if (st) return st;
   if (nsl <= 4) {
     hipLaunchKernelGGL(chi2_finish_kernel, dim3(1), dim3(kReduceThreads), 0, ctx->stream,
                        ctx->partials, nb, ctx->slice_tab, nsl, normalize, ctx->dvalues + slot);

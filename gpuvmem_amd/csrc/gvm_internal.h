@@ -147,6 +147,16 @@ gvm_status finish_reductions(gvm_ctx* ctx, int nslots, int nblocks);
 gvm_status finish_reductions_mode(gvm_ctx* ctx, int nslots, int nblocks, int is_max);
 // The same, writing the nslots results to `out` (device) instead of dscalars.
 gvm_status finish_reductions_to(gvm_ctx* ctx, int nslots, int nblocks, double* out);
+// Block partials of n fused prior terms in partial rows row0.. (blocks per
+// row in *nb_out), and the same with its finish into dvalues[first_slot..].
+gvm_status priors_value_partials(gvm_ctx* ctx, const float* I, const gvm_prior_term* terms,
+                                 int n, int row0, int* nb_out);
+gvm_status priors_value_partials_finish(gvm_ctx* ctx, const float* I,
+                                        const gvm_prior_term* terms, int n, int first_slot);
+// chi2 into dvalues[chi2_slot] and n prior terms into dvalues[first_slot..]
+// with one finishing launch (for up to 4 slices).
+gvm_status values_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int chi2_slot,
+                          const gvm_prior_term* terms, int n, int first_slot);
 // Enqueues the chi2 evaluation; 0.5 * sum over slices lands in dvalues[slot].
 gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot);
 // Copies dvalues[0, n) to the host (one synchronisation).

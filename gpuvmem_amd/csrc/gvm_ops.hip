@@ -21,6 +21,11 @@ namespace gvm {
 // ---------------------------------------------------------------------------
 enum { PRIOR_ENTROPY = 0, PRIOR_TSV = 1, PRIOR_LAPLACIAN = 2 };
 
+// Row and column of pixel k of an N x N image; N is a power of two
+// (gvm_create), so no 64-bit division.
+__device__ __forceinline__ int pix_row(long k, int N) { return (int)(k >> (31 - __clz(N))); }
+__device__ __forceinline__ int pix_col(long k, int N) { return (int)k & (N - 1); }
+
 // calculateS / calculateTSV / calculateL for pixel (i, j) of image `im`.
 __device__ __forceinline__ float prior_term(int kind, const float* __restrict__ im, int N, int i,
                                             int j, float G, float eta) {
@@ -73,7 +78,8 @@ __global__ __launch_bounds__(256) void prior_value_kernel(int kind, const float*
   double acc = 0.0;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
        k += (long)gridDim.x * blockDim.x) {
-    if (noise[k] < noise_cut) acc += (double)prior_term(kind, im, N, (int)(k / N), (int)(k % N), G, eta);
+    if (noise[k] < noise_cut)
+      acc += (double)prior_term(kind, im, N, pix_row(k, N), pix_col(k, N), G, eta);
   }
   const double tot = block_sum(acc, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
@@ -87,7 +93,7 @@ __global__ __launch_bounds__(256) void prior_grad_kernel(int kind, const float* 
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
        k += (long)gridDim.x * blockDim.x) {
     float d = 0.0f;
-    if (noise[k] < noise_cut) d = prior_grad(kind, im, N, (int)(k / N), (int)(k % N), G, eta);
+    if (noise[k] < noise_cut) d = prior_grad(kind, im, N, pix_row(k, N), pix_col(k, N), G, eta);
     dgi[k] = d * lambda;
   }
 }
@@ -117,7 +123,7 @@ __global__ __launch_bounds__(256) void priors_value_kernel(PriorTermsDev T,
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
        k += (long)gridDim.x * blockDim.x) {
     if (!(noise[k] < noise_cut)) continue;
-    const int i = (int)(k / N), j = (int)(k % N);
+    const int i = pix_row(k, N), j = pix_col(k, N);
 #pragma unroll
     for (int t = 0; t < GVM_MAX_FUSED_PRIORS; t++)
       if (t < T.n && T.t[t].active)
@@ -147,7 +153,7 @@ __global__ __launch_bounds__(256) void dphi_assemble_kernel(
   const long total = n * image_count;
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < total;
        k += (long)gridDim.x * blockDim.x) {
-    const int img = (int)(k / n);
+    const int img = (int)(k >> (2 * (31 - __clz(N))));  // k / n
     const long kk = k - (long)img * n;
     float r = 0.0f;
     if (chi2g) r = image_count == 1 ? 0.0f + chi2g[k] : chi2g[k];
@@ -160,7 +166,7 @@ __global__ __launch_bounds__(256) void dphi_assemble_kernel(
       if (T.t[t].active) {
         float d = 0.0f;
         if (inside)
-          d = prior_grad(T.t[t].kind, I + n * T.t[t].index, N, (int)(kk / N), (int)(kk % N),
+          d = prior_grad(T.t[t].kind, I + n * T.t[t].index, N, pix_row(kk, N), pix_col(kk, N),
                          T.t[t].prior, T.t[t].eta);
         ds = d * T.t[t].lambda;
       }
@@ -428,25 +434,41 @@ extern "C" gvm_status gvm_laplacian_grad(gvm_ctx* ctx, const float* I, float* dg
   return prior_grad(ctx, PRIOR_LAPLACIAN, I, dgi, index, 0.f, 0.f, lambda);
 }
 
+namespace gvm {
+// The block partials of n terms in partial rows row0 .. row0 + n - 1.
+gvm_status priors_value_partials(gvm_ctx* ctx, const float* I, const gvm_prior_term* terms,
+                                 int n, int row0, int* nb_out) {
+  PriorTermsDev T;
+  gvm_status st = check_terms(ctx, terms, n, &T);
+  if (st) return st;
+  const int N = ctx->g.N;
+  const int nb = reduce_grid((long)N * N);
+  *nb_out = nb;
+  if (n == 0) return GVM_OK;
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(priors_value_kernel, dim3(nb), dim3(256), 0, ctx->stream, T, I, ctx->noise,
+                     ctx->g.noise_cut, N, ctx->partials + (long)row0 * kReduceBlocks);
+  GVM_CHECK_LAUNCH();
+  return GVM_OK;
+}
+
+gvm_status priors_value_partials_finish(gvm_ctx* ctx, const float* I,
+                                        const gvm_prior_term* terms, int n, int first_slot) {
+  int nb = 0;
+  gvm_status st = priors_value_partials(ctx, I, terms, n, 0, &nb);
+  if (st || n == 0) return st;
+  // one block per term: finish_kernel's order, into consecutive value slots
+  return finish_reductions_to(ctx, n, nb, ctx->dvalues + first_slot);
+}
+}  // namespace gvm
+
 extern "C" gvm_status gvm_priors_value_enqueue(gvm_ctx* ctx, const float* I,
                                                const gvm_prior_term* terms, int n,
                                                int first_slot) {
   GVM_REQUIRE(ctx && I, "gvm_priors_value_enqueue: null argument");
-  PriorTermsDev T;
-  gvm_status st = check_terms(ctx, terms, n, &T);
-  if (st) return st;
-  GVM_REQUIRE(first_slot >= 0 && first_slot + n <= kValueSlots,
+  GVM_REQUIRE(first_slot >= 0 && n >= 0 && first_slot + n <= kValueSlots,
               "gvm_priors_value_enqueue: value slots out of range");
-  if (n == 0) return GVM_OK;
-  GVM_HIP_TRY(hipSetDevice(ctx->device));
-  const int N = ctx->g.N;
-  const long np = (long)N * N;
-  const int nb = reduce_grid(np);
-  hipLaunchKernelGGL(priors_value_kernel, dim3(nb), dim3(256), 0, ctx->stream, T, I, ctx->noise,
-                     ctx->g.noise_cut, N, ctx->partials);
-  GVM_CHECK_LAUNCH();
-  // one block per term: finish_kernel's order, into consecutive value slots
-  return finish_reductions_to(ctx, n, nb, ctx->dvalues + first_slot);
+  return priors_value_partials_finish(ctx, I, terms, n, first_slot);
 }
 
 extern "C" gvm_status gvm_dphi_assemble(gvm_ctx* ctx, const float* I, const float* chi2_grad,

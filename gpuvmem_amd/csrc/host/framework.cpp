@@ -179,14 +179,17 @@ void ObjectiveFunction::enqueueValues(float* p, std::vector<char>& queued) {
     for (size_t k = 0; k < fis.size(); k++) queued[k] = fis[k]->enqueueValue(p, (int)k) ? 1 : 0;
     return;
   }
-  queued[0] = fis[0]->enqueueValue(p, 0) ? 1 : 0;
+  // Chi2 and the priors with one finishing launch (what Chi2::enqueueValue
+  // and the priors' enqueueValue would give, in one pass over the image)
+  queued[0] = 1;
   std::vector<gvm_prior_term> terms(fis.size() - 1);
   for (size_t k = 1; k < fis.size(); k++) {
     fis[k]->priorTerm(&terms[k - 1], false);
     queued[k] = terms[k - 1].active ? 1 : 0;
   }
-  check(gvm_priors_value_enqueue(gvm::ctx(), p, terms.data(), (int)terms.size(), 1),
-        "gvm_priors_value_enqueue");
+  check(gvm_values_enqueue(gvm::ctx(), p, globals().flag_opt, fis[0]->getNormalize() ? 1 : 0, 0,
+                           terms.data(), (int)terms.size(), 1),
+        "gvm_values_enqueue");
 }
 
 double ObjectiveFunction::calcFunctionD(float* p) {

@@ -180,6 +180,7 @@ HIP_API = {
     "gvm_laplacian_grad": [_P, _P, _P, _I, _F],
     "gvm_add_to_dphi": [_P, _P, _P, _I],
     "gvm_priors_value_enqueue": [_P, _P, _P, _I, _I],
+    "gvm_values_enqueue": [_P, _P, _I, _I, _I, _P, _I, _I],
     "gvm_dphi_assemble": [_P, _P, _P, _P, _P, _P, _I],
     "gvm_vec_zero": [_P, _P, _L],
     "gvm_vec_copy": [_P, _P, _P, _L],

@@ -340,6 +340,12 @@ typedef struct {
  * one pass over the image for all terms. */
 gvm_status gvm_priors_value_enqueue(gvm_ctx* ctx, const float* I_dev, const gvm_prior_term* terms,
                                     int n, int first_slot);
+/* gvm_chi2_enqueue into chi2_slot followed by gvm_priors_value_enqueue into
+ * [first_slot, first_slot + n), with one finishing launch for all the
+ * values (for up to 4 slices; more take chi2's own finishing launches).
+ * Same values bit for bit. */
+gvm_status gvm_values_enqueue(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize,
+                              int chi2_slot, const gvm_prior_term* terms, int n, int first_slot);
 /* ObjectiveFunction's gradient assembly after Chi2::calcGi in one pass:
  * dphi = chi2_grad (NULL: zero), then dphi[term.image_to_add] += lambda *
  * dterm for each term in order, then xi = dphi; every buffer
