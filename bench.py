@@ -335,6 +335,47 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
     return out
 
 
+def bench_unet_step(steps: int, warmup: int, levels: int = 6, base: int = 32, batch: int = 8,
+                    npix: int = 512) -> dict:
+    """Training steps of the warm-start U-Net on synthetic batches: forward,
+    MSE loss, backward and one Adam update per step, on the project's own
+    kernels (FP32-MFMA implicit-GEMM convolutions, BF16 weights)."""
+    from gpuvmem_amd.device import Context, DeviceBuffer
+    from gpuvmem_amd.problem import make_problem
+    from gpuvmem_amd.warmstart import UNet
+
+    ctx = Context(make_problem(0, npix=64), add_data=False)
+    net = UNet(ctx, levels=levels, base=base, batch=batch, H=npix, W=npix, seed=1, bf16=True)
+    rng = np.random.default_rng(2)
+    net.set_input(rng.standard_normal((batch, 1, npix, npix)).astype(np.float32))
+    target = DeviceBuffer.from_array(rng.standard_normal((batch, 1, npix, npix)).astype(np.float32))
+
+    def step():
+        net.forward()
+        net.loss(target)
+        net.backward()
+        net.adam(lr=1e-4)
+
+    for _ in range(max(1, warmup)):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    el = (time.perf_counter() - t0) / steps
+    out = {"metric": "warm-start network training step (forward + MSE + backward + Adam)",
+           "value": el * 1e3, "unit": "ms per step", "higher_is_better": False, "n_gpus": 1,
+           "steps": steps, "data": "synthetic",
+           "config": {"levels": levels, "base_channels": base, "batch": batch,
+                      "image": [npix, npix], "parameters": net.nparams,
+                      "weights": "bf16 (fp32 master)",
+                      "convolutions": "implicit GEMM on v_mfma_f32_32x32x2_f32"}}
+    net.close()
+    ctx.close()
+    return out
+
+
 def profiled_traffic(mode: str, nvis: int, split_vis: int):
     """HBM bytes per launch of the adjoint GEMM from the committed rocprofv3
     counter summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
@@ -578,6 +619,10 @@ def main() -> int:
                     help="warm start: sky seeds [A, B) of the pairs that --ws-save generates")
     ap.add_argument("--ws-load", nargs="+", metavar="FILE",
                     help="warm start: train on the pairs of these files instead of generating")
+    ap.add_argument("--unet-step", action="store_true",
+                    help="time training steps (forward, MSE, backward, Adam) of the warm-start "
+                         "network at its size: 6 levels, 32 base channels, 512^2, batch 8, BF16 "
+                         "weights")
     ap.add_argument("--gridding", action="store_true",
                     help="time the device weighting and gridder and the gridded objective step")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -591,6 +636,9 @@ def main() -> int:
     from gpuvmem_amd.device import Context, Objective, fw_config
     from gpuvmem_amd.problem import make_mfs_problem, make_problem
 
+    if args.unet_step:
+        print(json.dumps(bench_unet_step(args.steps, args.warmup)), flush=True)
+        return 0
     if args.warm_start:
         if args.ws_no_graphs:
             from gpuvmem_amd import warmstart

@@ -309,6 +309,7 @@ extern "C" gvm_status gvm_destroy(gvm_ctx* ctx) {
   if (ctx->hscalars) (void)hipHostFree(ctx->hscalars);
   (void)hipFree(ctx->dvalues);
   if (ctx->hvalues) (void)hipHostFree(ctx->hvalues);
+  (void)hipFree(ctx->nn_part);
   for (hipEvent_t& e : ctx->tev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

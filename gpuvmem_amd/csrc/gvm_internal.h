@@ -117,6 +117,9 @@ struct gvm_ctx {
   // epoch changes whenever recorded device pointers or launches go stale
   bool capturing = false;
   long epoch = 0;
+  // network workspace: the weight-gradient partials of a convolution
+  float* nn_part = nullptr;
+  size_t nn_part_elems = 0;
   // timing of the adjoint GEMM: a ring of event pairs recorded around each
   // eager launch and read back lazily (when the ring is full or the timing
   // is queried), so an evaluation never waits on the host for its timing

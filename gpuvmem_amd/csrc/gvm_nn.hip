@@ -2,11 +2,10 @@
 // optional ReLU, 2x2 average pooling, nearest 2x upsampling into a channel
 // block, channel copies, the MSE loss and Adam, forward and backward.
 //
-// Direct convolutions, one thread per output element (forward) or input
-// element (input gradient).  Parameter gradients take one workgroup per
-// (output channel, input channel) pair with the K*K taps in registers,
-// accumulated in FP64 and reduced in a fixed order, so training is
-// reproducible bit for bit.  Layouts are NCHW.
+// Convolutions are implicit GEMMs on the FP32 MFMA units (forward, input
+// gradient, and weight gradient in pixel ranges whose partials are summed in
+// FP64 in a fixed order); bias gradients and the loss are FP64 fixed-order
+// reductions, so training is reproducible bit for bit.  Layouts are NCHW.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,126 +31,193 @@ __device__ __forceinline__ float relu_grad(const float* out, const float* dout, 
   return relu ? (out[i] > 0.0f ? g : 0.0f) : g;
 }
 
+// ---------------------------------------------------------------------------
+// Convolutions as implicit GEMMs on the FP32 MFMA units
+// (v_mfma_f32_32x32x2_f32: FP32 products, FP32 accumulation).
+//
+//   forward      out[co][q]   = sum_{ci,tap} w[co][ci][tap] in[ci][q + tap]
+//                M = Co, N = pixels (b, y, x), K = Ci K K
+//   input grad   din[ci][q]   = sum_{co,tap} w[co][ci][tap] g[co][q - tap]
+//                M = Ci, N = pixels, K = Co K K
+//   weight grad  dw[co][ci,tap] = sum_q g[co][q] in[ci][q + tap]
+//                M = Co, N = Ci K K, K = pixels, split in ranges of
+//                kWgradPixels whose FP32 partials are summed in FP64 in a
+//                fixed order (bitwise reproducible)
+//
+// tap (ky, kx) shifts a pixel by (ky - K/2, kx - K/2), zero outside the
+// image; g is the output gradient masked by the ReLU of the forward output.
+// A workgroup (4 waves) computes a 64 x 64 tile of M x N, each wave a 32 x
+// 32 block; K advances in chunks of 16 staged in LDS (k-major, so the MFMA
+// operand reads of a wave are consecutive), the next chunk's values loaded
+// into registers while the current one is multiplied.
+// ---------------------------------------------------------------------------
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+enum { CONV_FWD = 0, CONV_BWD_DATA = 1, CONV_BWD_WEIGHT = 2 };
+constexpr int kCTile = 64;
+constexpr int kCK = 16;
+constexpr int kWgradPixels = 8192;
+
+template <class WT>
+struct ConvArgs {
+  const float* in;    // [B][Ci][H][W]
+  const WT* w;        // [Co][Ci][K][K]
+  const float* bias;  // [Co] (forward)
+  const float* out;   // forward output (ReLU mask), or NULL
+  const float* dout;  // [B][Co][H][W] gradient at the activated output
+  float* dst;         // out (forward), din (input grad) or the weight-grad partials
+  int B, Ci, Co, H, W, relu;
+};
+
 template <int K, class WT>
-__global__ __launch_bounds__(kT) void conv_fwd_kernel(const float* __restrict__ in,
-                                                      const WT* __restrict__ w,
-                                                      const float* __restrict__ bias,
-                                                      float* __restrict__ out, int B, int Ci,
-                                                      int Co, int H, int W, int relu) {
+__device__ __forceinline__ float conv_in_at(const ConvArgs<WT>& a, int ci, int q, int tap) {
+  // in[b][ci][y + ky - P][x + kx - P] for pixel q = (b, y, x), 0 outside
   constexpr int P = K / 2;
-  const long HW = (long)H * W;
-  const long n = (long)B * Co * HW;
-  for (long o = blockIdx.x * (long)kT + threadIdx.x; o < n; o += (long)gridDim.x * kT) {
-    const int x = (int)(o % W);
-    const int y = (int)((o / W) % H);
-    const int co = (int)((o / HW) % Co);
-    const int b = (int)(o / (HW * Co));
-    const float* __restrict__ src = in + (long)b * Ci * HW;
-    const WT* __restrict__ wc = w + (long)co * Ci * K * K;
-    float acc = bias[co];
-    for (int ci = 0; ci < Ci; ci++) {
-      const float* __restrict__ s = src + (long)ci * HW;
-      const WT* __restrict__ wk = wc + ci * K * K;
-#pragma unroll
-      for (int ky = 0; ky < K; ky++) {
-        const int yy = y + ky - P;
-        if (yy < 0 || yy >= H) continue;
-#pragma unroll
-        for (int kx = 0; kx < K; kx++) {
-          const int xx = x + kx - P;
-          if (xx < 0 || xx >= W) continue;
-          acc = fmaf(wload(wk, ky * K + kx), s[(long)yy * W + xx], acc);
-        }
-      }
-    }
-    out[o] = relu ? fmaxf(acc, 0.0f) : acc;
-  }
+  const int HW = a.H * a.W;
+  const int b = q / HW, p = q - b * HW;
+  const int yy = p / a.W + tap / K - P, xx = p % a.W + tap % K - P;
+  if (yy < 0 || yy >= a.H || xx < 0 || xx >= a.W) return 0.0f;
+  return a.in[((long)b * a.Ci + ci) * HW + (long)yy * a.W + xx];
 }
 
-// din[b][ci][y][x] = sum_co sum_k w[co][ci][k] g[b][co][y - ky + P][x - kx + P]
-template <int K, class WT>
-__global__ __launch_bounds__(kT) void conv_bwd_data_kernel(const WT* __restrict__ w,
-                                                           const float* __restrict__ out,
-                                                           const float* __restrict__ dout,
-                                                           float* __restrict__ din, int B,
-                                                           int Ci, int Co, int H, int W,
-                                                           int relu) {
-  constexpr int P = K / 2;
-  const long HW = (long)H * W;
-  const long n = (long)B * Ci * HW;
-  for (long o = blockIdx.x * (long)kT + threadIdx.x; o < n; o += (long)gridDim.x * kT) {
-    const int x = (int)(o % W);
-    const int y = (int)((o / W) % H);
-    const int ci = (int)((o / HW) % Ci);
-    const int b = (int)(o / (HW * Ci));
-    float acc = 0.0f;
-    for (int co = 0; co < Co; co++) {
-      const long base = ((long)b * Co + co) * HW;
-      const WT* __restrict__ wk = w + ((long)co * Ci + ci) * K * K;
-#pragma unroll
-      for (int ky = 0; ky < K; ky++) {
-        const int yy = y - ky + P;
-        if (yy < 0 || yy >= H) continue;
-#pragma unroll
-        for (int kx = 0; kx < K; kx++) {
-          const int xx = x - kx + P;
-          if (xx < 0 || xx >= W) continue;
-          acc = fmaf(wload(wk, ky * K + kx), relu_grad(out, dout, base + (long)yy * W + xx, relu),
-                     acc);
-        }
-      }
-    }
-    din[o] = acc;
-  }
+template <class WT>
+__device__ __forceinline__ float conv_g_at(const ConvArgs<WT>& a, int co, int q) {
+  // the masked output gradient at pixel q = (b, y, x) of output channel co
+  const int HW = a.H * a.W;
+  const int b = q / HW, p = q - b * HW;
+  const long i = ((long)b * a.Co + co) * HW + p;
+  return relu_grad(a.out, a.dout, i, a.relu);
 }
 
-// One workgroup per (co, ci): dw[co][ci][ky][kx] = sum_b,y,x g[b][co][y][x]
-// in[b][ci][y + ky - P][x + kx - P], FP64 per thread, fixed-order tree.
-template <int K>
-__global__ __launch_bounds__(kT) void conv_bwd_weight_kernel(const float* __restrict__ in,
-                                                             const float* __restrict__ out,
-                                                             const float* __restrict__ dout,
-                                                             float* __restrict__ dw, int B,
-                                                             int Ci, int Co, int H, int W,
-                                                             int relu) {
-  constexpr int P = K / 2;
+template <int MODE, int K, class WT>
+__device__ __forceinline__ float conv_a(const ConvArgs<WT>& a, int m, int k, int M, int Kd) {
+  if (m >= M || k >= Kd) return 0.0f;
   constexpr int KK = K * K;
-  __shared__ double red[KK][kT];
-  const int co = blockIdx.x / Ci, ci = blockIdx.x % Ci;
-  const long HW = (long)H * W;
-  double acc[KK];
+  if constexpr (MODE == CONV_FWD) return wload(a.w, (long)m * Kd + k);
+  if constexpr (MODE == CONV_BWD_DATA) {
+    const int co = k / KK, tap = k - co * KK;  // w[co][ci = m][tap]
+    return wload(a.w, ((long)co * a.Ci + m) * KK + tap);
+  }
+  return conv_g_at(a, m, k);  // weight grad: g[co = m][pixel k]
+}
+
+template <int MODE, int K, class WT>
+__device__ __forceinline__ float conv_b(const ConvArgs<WT>& a, int k, int n, int Kd, int Nd) {
+  if (k >= Kd || n >= Nd) return 0.0f;
+  constexpr int KK = K * K;
+  if constexpr (MODE == CONV_FWD) {
+    const int ci = k / KK;
+    return conv_in_at<K>(a, ci, n, k - ci * KK);
+  }
+  if constexpr (MODE == CONV_BWD_DATA) {
+    // g[co][q - tap] with k = (co, tap)
+    constexpr int P = K / 2;
+    const int co = k / KK, tap = k - co * KK;
+    const int HW = a.H * a.W;
+    const int b = n / HW, p = n - b * HW;
+    const int yy = p / a.W - (tap / K - P), xx = p % a.W - (tap % K - P);
+    if (yy < 0 || yy >= a.H || xx < 0 || xx >= a.W) return 0.0f;
+    return relu_grad(a.out, a.dout, ((long)b * a.Co + co) * HW + (long)yy * a.W + xx, a.relu);
+  }
+  const int ci = n / KK;  // weight grad: in[ci][pixel k + tap], n = (ci, tap)
+  return conv_in_at<K>(a, ci, k, n - ci * KK);
+}
+
+template <int MODE, int K, class WT>
+__global__ __launch_bounds__(256) void conv_mfma_kernel(ConvArgs<WT> a) {
+  constexpr int KK = K * K;
+  const int HW = a.H * a.W;
+  const int npix = a.B * HW;
+  int M, Nd, Kd, kbeg;
+  if constexpr (MODE == CONV_FWD) {
+    M = a.Co; Nd = npix; Kd = a.Ci * KK; kbeg = 0;
+  } else if constexpr (MODE == CONV_BWD_DATA) {
+    M = a.Ci; Nd = npix; Kd = a.Co * KK; kbeg = 0;
+  } else {
+    M = a.Co; Nd = a.Ci * KK; kbeg = blockIdx.z * kWgradPixels;
+    Kd = min(npix, kbeg + kWgradPixels);
+  }
+  __shared__ float As[kCK][kCTile + 4];
+  __shared__ float Bs[kCK][kCTile + 4];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int m0 = blockIdx.y * kCTile, n0 = blockIdx.x * kCTile;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  // loader coordinates: 4 A and 4 B values per thread per chunk.  The
+  // weight grad reads along pixels (its K) for both operands, the others
+  // along k for A and along pixels (N) for B.
+  float ra[4], rb[4];
+  auto load = [&](int k0) {
 #pragma unroll
-  for (int k = 0; k < KK; k++) acc[k] = 0.0;
-  const long n = (long)B * HW;
-  for (long i = threadIdx.x; i < n; i += kT) {
-    const int b = (int)(i / HW);
-    const long p = i % HW;
-    const int y = (int)(p / W), x = (int)(p % W);
-    const float g = relu_grad(out, dout, ((long)b * Co + co) * HW + p, relu);
-    if (g == 0.0f) continue;
-    const float* __restrict__ s = in + ((long)b * Ci + ci) * HW;
-#pragma unroll
-    for (int ky = 0; ky < K; ky++) {
-      const int yy = y + ky - P;
-      if (yy < 0 || yy >= H) continue;
-#pragma unroll
-      for (int kx = 0; kx < K; kx++) {
-        const int xx = x + kx - P;
-        if (xx < 0 || xx >= W) continue;
-        acc[ky * K + kx] += (double)g * (double)s[(long)yy * W + xx];
+    for (int j = 0; j < 4; j++) {
+      const int e = t + 256 * j;
+      const int am = e / kCK, ak = e % kCK;
+      ra[j] = conv_a<MODE, K>(a, m0 + am, k0 + ak, M, Kd);
+      int bk, bn;
+      if constexpr (MODE == CONV_BWD_WEIGHT) {
+        bk = e % kCK;
+        bn = e / kCK;
+      } else {
+        bk = e / kCTile;
+        bn = e % kCTile;
       }
+      rb[j] = conv_b<MODE, K>(a, k0 + bk, n0 + bn, Kd, Nd);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int e = t + 256 * j;
+      As[e % kCK][e / kCK] = ra[j];
+      if constexpr (MODE == CONV_BWD_WEIGHT)
+        Bs[e % kCK][e / kCK] = rb[j];
+      else
+        Bs[e / kCTile][e % kCTile] = rb[j];
+    }
+  };
+  f32x16_t acc;
+#pragma unroll
+  for (int e = 0; e < 16; e++) acc[e] = 0.0f;
+  const int lr = lane & 31, lh = lane >> 5;
+  load(kbeg);
+  for (int k0 = kbeg; k0 < Kd; k0 += kCK) {
+    __syncthreads();  // the previous chunk's reads are done
+    store();
+    __syncthreads();
+    if (k0 + kCK < Kd) load(k0 + kCK);
+#pragma unroll
+    for (int s = 0; s < kCK / 2; s++)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(As[2 * s + lh][wm + lr], Bs[2 * s + lh][wn + lr],
+                                                 acc, 0, 0, 0);
+  }
+  // 32 x 32 accumulator layout: lane l, register e -> row 8 (e / 4) + 4 (l / 32)
+  // + e % 4 of the wave's block, column l % 32
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    const int m = m0 + wm + 8 * (e >> 2) + 4 * lh + (e & 3);
+    const int n = n0 + wn + lr;
+    if (m >= M || n >= Nd) continue;
+    if constexpr (MODE == CONV_BWD_WEIGHT) {
+      a.dst[((long)blockIdx.z * M + m) * Nd + n] = acc[e];
+    } else {
+      const int b = n / HW, p = n - b * HW;
+      float v = acc[e];
+      if constexpr (MODE == CONV_FWD) {
+        v = v + a.bias[m];
+        if (a.relu) v = fmaxf(v, 0.0f);
+      }
+      a.dst[((long)b * M + m) * HW + p] = v;
     }
   }
-#pragma unroll
-  for (int k = 0; k < KK; k++) red[k][threadIdx.x] = acc[k];
-  __syncthreads();
-  for (int h = kT / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h)
-#pragma unroll
-      for (int k = 0; k < KK; k++) red[k][threadIdx.x] += red[k][threadIdx.x + h];
-    __syncthreads();
+}
+
+// dw = the FP64 sum of the weight-grad partials over the pixel ranges, in
+// range order.
+__global__ __launch_bounds__(kT) void wgrad_reduce_kernel(const float* __restrict__ part, int nsplit,
+                                                          long n, float* __restrict__ dw) {
+  for (long i = blockIdx.x * (long)kT + threadIdx.x; i < n; i += (long)gridDim.x * kT) {
+    double s = 0.0;
+    for (int r = 0; r < nsplit; r++) s += (double)part[(long)r * n + i];
+    dw[i] = (float)s;
   }
-  if ((int)threadIdx.x < KK) dw[((long)co * Ci + ci) * KK + threadIdx.x] = (float)red[threadIdx.x][0];
 }
 
 // One workgroup per output channel: dbias[co] = sum g[b][co][:, :].
@@ -337,13 +403,16 @@ gvm_status conv_fwd_launch(gvm_ctx* ctx, const float* in, const WT* w, const flo
   GVM_REQUIRE(ctx && in && w && bias && out, "gvm_nn_conv_fwd: null argument");
   GVM_REQUIRE(K == 1 || K == 3, "gvm_nn_conv_fwd: K must be 1 or 3");
   GVM_REQUIRE(dims_ok(B, Ci, Co, H, W), "gvm_nn_conv_fwd: bad sizes");
-  const long n = (long)B * Co * H * W;
+  GVM_REQUIRE((long)B * H * W < (1L << 31) && (long)Ci * K * K < (1L << 24) &&
+                  (long)Co * K * K < (1L << 24),
+              "gvm_nn_conv_fwd: sizes beyond the kernels' 32-bit indices");
+  const ConvArgs<WT> a{in, w, bias, nullptr, nullptr, out, B, Ci, Co, H, W, relu};
+  const dim3 grid((unsigned)((B * H * W + kCTile - 1) / kCTile),
+                  (unsigned)((Co + kCTile - 1) / kCTile));
   if (K == 3)
-    hipLaunchKernelGGL((conv_fwd_kernel<3, WT>), dim3(nblocks(n)), dim3(kT), 0, ctx->stream, in, w,
-                       bias, out, B, Ci, Co, H, W, relu);
+    hipLaunchKernelGGL((conv_mfma_kernel<CONV_FWD, 3, WT>), grid, dim3(256), 0, ctx->stream, a);
   else
-    hipLaunchKernelGGL((conv_fwd_kernel<1, WT>), dim3(nblocks(n)), dim3(kT), 0, ctx->stream, in, w,
-                       bias, out, B, Ci, Co, H, W, relu);
+    hipLaunchKernelGGL((conv_mfma_kernel<CONV_FWD, 1, WT>), grid, dim3(256), 0, ctx->stream, a);
   return launched("gvm_nn_conv_fwd");
 }
 
@@ -356,21 +425,48 @@ gvm_status conv_bwd_launch(gvm_ctx* ctx, const float* in, const WT* w, const flo
   GVM_REQUIRE(K == 1 || K == 3, "gvm_nn_conv_bwd: K must be 1 or 3");
   GVM_REQUIRE(dims_ok(B, Ci, Co, H, W), "gvm_nn_conv_bwd: bad sizes");
   GVM_REQUIRE((long)Co * Ci < (1L << 31), "gvm_nn_conv_bwd: too many channel pairs");
-  const long nin = (long)B * Ci * H * W;
+  GVM_REQUIRE((long)B * H * W < (1L << 31) && (long)Ci * K * K < (1L << 24) &&
+                  (long)Co * K * K < (1L << 24),
+              "gvm_nn_conv_bwd: sizes beyond the kernels' 32-bit indices");
+  const int npix = B * H * W;
   if (din) {
+    const ConvArgs<WT> a{in, w, nullptr, out, dout, din, B, Ci, Co, H, W, relu};
+    const dim3 grid((unsigned)((npix + kCTile - 1) / kCTile),
+                    (unsigned)((Ci + kCTile - 1) / kCTile));
     if (K == 3)
-      hipLaunchKernelGGL((conv_bwd_data_kernel<3, WT>), dim3(nblocks(nin)), dim3(kT), 0,
-                         ctx->stream, w, out, dout, din, B, Ci, Co, H, W, relu);
+      hipLaunchKernelGGL((conv_mfma_kernel<CONV_BWD_DATA, 3, WT>), grid, dim3(256), 0, ctx->stream,
+                         a);
     else
-      hipLaunchKernelGGL((conv_bwd_data_kernel<1, WT>), dim3(nblocks(nin)), dim3(kT), 0,
-                         ctx->stream, w, out, dout, din, B, Ci, Co, H, W, relu);
+      hipLaunchKernelGGL((conv_mfma_kernel<CONV_BWD_DATA, 1, WT>), grid, dim3(256), 0, ctx->stream,
+                         a);
   }
-  if (K == 3)
-    hipLaunchKernelGGL(conv_bwd_weight_kernel<3>, dim3(Co * Ci), dim3(kT), 0, ctx->stream, in, out,
-                       dout, dw, B, Ci, Co, H, W, relu);
-  else
-    hipLaunchKernelGGL(conv_bwd_weight_kernel<1>, dim3(Co * Ci), dim3(kT), 0, ctx->stream, in, out,
-                       dout, dw, B, Ci, Co, H, W, relu);
+  // weight gradient: FP32 partials per range of kWgradPixels pixels in the
+  // context's network workspace, then their FP64 sum in range order
+  const int nsplit = (npix + kWgradPixels - 1) / kWgradPixels;
+  const long nw = (long)Co * Ci * K * K;
+  const size_t need = (size_t)nsplit * nw;
+  if (need > ctx->nn_part_elems) {
+    GVM_REQUIRE(!ctx->capturing, "gvm_nn_conv_bwd: the workspace would grow under capture");
+    GVM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->nn_part) GVM_HIP_TRY(hipFree(ctx->nn_part));
+    ctx->nn_part = nullptr;
+    ctx->nn_part_elems = 0;
+    GVM_HIP_TRY(hipMalloc((void**)&ctx->nn_part, sizeof(float) * need));
+    ctx->nn_part_elems = need;
+  }
+  {
+    const ConvArgs<WT> a{in, w, nullptr, out, dout, ctx->nn_part, B, Ci, Co, H, W, relu};
+    const dim3 grid((unsigned)((Ci * K * K + kCTile - 1) / kCTile),
+                    (unsigned)((Co + kCTile - 1) / kCTile), (unsigned)nsplit);
+    if (K == 3)
+      hipLaunchKernelGGL((conv_mfma_kernel<CONV_BWD_WEIGHT, 3, WT>), grid, dim3(256), 0,
+                         ctx->stream, a);
+    else
+      hipLaunchKernelGGL((conv_mfma_kernel<CONV_BWD_WEIGHT, 1, WT>), grid, dim3(256), 0,
+                         ctx->stream, a);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(nblocks(nw)), dim3(kT), 0, ctx->stream,
+                       ctx->nn_part, nsplit, nw, dw);
+  }
   hipLaunchKernelGGL(conv_bwd_bias_kernel, dim3(Co), dim3(kT), 0, ctx->stream, out, dout, dbias, B,
                      Co, (long)H * W, relu);
   return launched("gvm_nn_conv_bwd");
