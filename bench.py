@@ -828,10 +828,11 @@ def plain_fp32_step(p, steps: int, warmup: int, graphs: bool) -> dict:
     ctx.sync()
     el = time.perf_counter() - t0
     ms, n = ctx.adjoint_time_ms()
+    split_vis = ctx.split_vis()
     obj.close()
     ctx.close()
     return {"evals_per_s": steps / el, "ms_per_step": el / steps * 1e3, "steps": steps,
-            "adjoint_kernel_ms": ms if n else None, "split_vis": 3072, "graphs": bool(graphs)}
+            "adjoint_kernel_ms": ms if n else None, "split_vis": split_vis, "graphs": bool(graphs)}
 
 
 def compact_tti(r: dict) -> dict:

@@ -109,7 +109,7 @@ struct gvm_ctx {
   int nsplit_total = 0;
   int split_mode = -1;          // adjoint mode the table was built for (-1: stale)
   int adjoint_mode = GVM_ADJOINT_BF16X3;
-  long split_vis = 3072;        // target visibilities per adjoint range (K tile)
+  long split_vis = 6144;        // target visibilities per adjoint range (K tile)
   int fft_mode = 0;             // FFT batching: 0 automatic, 1 wide, 2 narrow
   std::vector<gvm::Slice> slices;
   bool have_residuals = false;

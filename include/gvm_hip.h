@@ -475,7 +475,7 @@ gvm_status gvm_timing_adjoint_ms(gvm_ctx* ctx, double* avg_ms, long* count);
  * adjoint accumulates one 256 x 256 output tile over a range of about
  * `split_vis` visibilities in FP32 (the ranges are summed in FP64): shorter
  * ranges keep the FP32 accumulation short, longer ones write fewer partial
- * tiles.  Default 3072; allowed 256 .. 16384. */
+ * tiles.  Default 6144; allowed 256 .. 16384. */
 gvm_status gvm_set_split_vis(gvm_ctx* ctx, long split_vis);
 /* Batching of the image-to-grid transform (no counterpart in the reference,
  * which calls cuFFT): 0 automatic (default), 1 wide blocks (many rows or

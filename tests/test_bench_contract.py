@@ -14,14 +14,15 @@ import bench  # noqa: E402
 
 NVIS = 1_764_000
 # the GEMM writes one FP32 1024 x 1024 partial image per visibility range: ranges of
-# 3072 visibilities, their count padded to a multiple of 8 (575 -> 576)
-RANGES = -(-(-(-NVIS // 3072)) // 8) * 8
+# 6144 visibilities (the default), their count padded to a multiple of 8 (288)
+SPLIT = 6144
+RANGES = -(-(-(-NVIS // SPLIT)) // 8) * 8
 PARTIAL_BYTES = RANGES * 1024 * 1024 * 4
 
 
 @pytest.mark.parametrize("mode", ["bf16x3", "f16x3"])
 def test_traffic_summary_for_each_mfma_form(mode):
-    traffic, source = bench.profiled_traffic(mode, NVIS, 3072)
+    traffic, source = bench.profiled_traffic(mode, NVIS, SPLIT)
     assert source is not None, f"no counter summary for {mode} at {NVIS} visibilities"
     assert (ROOT / "profiles" / source).exists()
     # at least the partial tiles it writes, at most a few percent more than that plus reads
@@ -31,7 +32,7 @@ def test_traffic_summary_for_each_mfma_form(mode):
 
 
 def test_no_summary_for_an_unprofiled_size_or_range_length():
-    assert bench.profiled_traffic("bf16x3", NVIS + 1, 3072) == (None, None)
+    assert bench.profiled_traffic("bf16x3", NVIS + 1, SPLIT) == (None, None)
     # another range length writes another number of partial images
     assert bench.profiled_traffic("bf16x3", NVIS, 1536) == (None, None)
 

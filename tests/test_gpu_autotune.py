@@ -79,7 +79,8 @@ def full():
 
 @pytest.mark.parametrize("mode", ["bf16x3", "f16x3"])
 def test_full_size_gradient_at_range_lengths(full, mode):
-    """The shortest and longest default candidates and the default (3072)."""
+    """The shortest and longest default candidates, the previous default
+    (3072) and the default (6144)."""
     p = full
     print(f"case: full size range lengths {mode} {p.M}^2 nvis={p.nvis}", flush=True)
     orc = oracle.Oracle(p)
@@ -93,7 +94,7 @@ def test_full_size_gradient_at_range_lengths(full, mode):
     ctx = Context(p, adjoint=mode)
     d = ctx.image(img)
     ctx.chi2(d)
-    for sv in (1536, 3072, 8192):
+    for sv in (1536, 3072, 6144, 8192):
         ctx.set_split_vis(sv)
         r = ctx.zeros(2 * p.M * p.N)
         ctx.dchi2(d, r)

@@ -231,7 +231,7 @@ def test_full_size_precision_forms_agree(full):
 
 
 def test_full_size_bf16x3_whole_image_against_fp64(full):
-    """The benchmark's form and range length (bf16x3, 3072 visibilities per
+    """The benchmark's form and range length (bf16x3, 6144 visibilities per
     range) over the whole 1024^2 image against the FP64 oracle's separable
     gradient (about 20 s of CPU on 16 threads)."""
     p = full
@@ -239,7 +239,7 @@ def test_full_size_bf16x3_whole_image_against_fp64(full):
     img = p.initial_image()
     img[0] = np.maximum(p.sky / p.fg_scale * 0.5, p.minpix).astype(np.float32)
     ctx = Context(p, adjoint="bf16x3")
-    assert ctx.split_vis() == 3072
+    assert ctx.split_vis() == 6144
     d = ctx.image(img)
     ctx.chi2(d)
     res = ctx.zeros(2 * p.M * p.N)
