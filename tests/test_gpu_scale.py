@@ -254,3 +254,25 @@ def test_full_size_bf16x3_whole_image_against_fp64(full):
     print(f"whole-image gradient rel L2 against FP64 {err:.3e}")
     assert err < GRAD_TOL
     assert np.all(g[1] == 0)
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "f32"])
+def test_tiny_weights_keep_the_adjoint_scale_finite(mode):
+    """Weights scaled so that the largest |c| = w |V_r| is about 2^-116: the
+    power-of-two scale that brings P to 2^15 would be 2^131, beyond FP32, and
+    without its clamp the zero amplitudes of off-grid samples (0 * inf) would
+    turn the gradient into NaN.  The gradient stays finite and within the bar."""
+    p = small_problem(npix=64, n_antennas=6, seed=9)
+    p.uvw = p.uvw.copy()
+    dxr = p.deltax * np.pi / 180.0
+    p.uvw[::5, 0] = -3.7 * p.M / (p.M * dxr) * np.sign(dxr)  # some samples off the grid
+    img = _rand_img(p, 7)
+    _, vr0, _ = oracle.Oracle(p).chi2(img)
+    w0 = oracle.Oracle(p).weight
+    cm0 = np.max(w0 * np.abs(vr0))
+    p.weight = (p.weight.astype(np.float64) * (2.0 ** -116 / cm0)).astype(np.float32)
+    print(f"case: tiny weights {mode}, max w {p.weight.max():.3e}", flush=True)
+    ctx = Context(p, adjoint=mode)
+    err = _check(ctx, oracle.Oracle(p), img)
+    ctx.close()
+    print(f"gradient rel L2 {err:.3e}")
