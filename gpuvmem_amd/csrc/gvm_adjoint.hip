@@ -341,14 +341,22 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
           bhi[cb] = *(const u16x8_t*)&sm.bh[buf][o];
           blo[cb] = *(const u16x8_t*)&sm.bl[buf][o];
         }
+        // The three products of a block are issued eight MFMAs apart (all
+        // blocks' hi*hi, then hi*lo, then lo*hi): back to back on one
+        // accumulator, each waited for the one before (3.5 % slower).  Each
+        // accumulator still sees the same sequence of products.
 #pragma unroll
         for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-          for (int cb = 0; cb < 2; cb++) {
-            acc[rb][cb] = mfma16(ahi[rb], bhi[cb], acc[rb][cb], PREC);
-            acc[rb][cb] = mfma16(ahi[rb], blo[cb], acc[rb][cb], PREC);
-            acc[rb][cb] = mfma16(alo[rb], bhi[cb], acc[rb][cb], PREC);
-          }
+          for (int cb = 0; cb < 2; cb++) acc[rb][cb] = mfma16(ahi[rb], bhi[cb], acc[rb][cb], PREC);
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+          for (int cb = 0; cb < 2; cb++) acc[rb][cb] = mfma16(ahi[rb], blo[cb], acc[rb][cb], PREC);
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+          for (int cb = 0; cb < 2; cb++) acc[rb][cb] = mfma16(alo[rb], bhi[cb], acc[rb][cb], PREC);
       }
     }
   };
