@@ -10,7 +10,7 @@
 //  1. fft_pass<rows>     clip2IWNoise in place on the caller's image, then
 //                        I_nu = clip(I0 (nu/nu0)^alpha), times PB and
 //                        fg_scale, then the row transforms (Stockham
-//                        radix-4/2 in LDS), times the column-frequency
+//                        radix-8/4 in LDS), times the column-frequency
 //                        phase-rotation factor;
 //  2. fft_pass<cols>     column transforms in LDS, times the row-frequency
 //                        phase-rotation factor, in place on the grid;
@@ -18,6 +18,8 @@
 //                        summed per block in FP64, and max |w V_r| for the
 //                        adjoint's FP16 scaling;
 //  4. finish             fixed-order FP64 sum of the block partials.
+// For up to four slices step 3 and the prior terms' value partials are one
+// launch (values_kernel), and step 4 with the priors' finish one more.
 // The polar form of c = w conj(V_r) that the adjoint reads is written by the
 // adjoint itself (polar_kernel), so value-only evaluations (line-search
 // probes) do not pay for it.
@@ -30,6 +32,7 @@
 #include <algorithm>
 
 #include "gvm_internal.h"
+#include "gvm_prior_terms.h"
 
 namespace gvm {
 
@@ -337,12 +340,15 @@ gvm_status model_grids(gvm_ctx* ctx, float* I, int nsl, int flag_opt) {
 // ---------------------------------------------------------------------------
 constexpr int kDegridThreads = 512;
 
-__global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
-    const float2* __restrict__ grids, int N, const SliceDev* __restrict__ tab,
-    double* __restrict__ partials, unsigned int* __restrict__ cmaxs) {
-  __shared__ double sh[kDegridThreads / 64];
-  const SliceDev& sd = tab[blockIdx.y];
-  const float2* __restrict__ grid = grids + (long)blockIdx.y * N * N;
+// One degrid block: block bx of nbx over slice `slice` (blockIdx.x and
+// gridDim.x of degrid_chi2_kernel), its partial into partials[slice][bx].
+__device__ __forceinline__ void degrid_block(const float2* __restrict__ grids, int N,
+                                             const SliceDev* __restrict__ tab,
+                                             double* __restrict__ partials,
+                                             unsigned int* __restrict__ cmaxs, int bx, int nbx,
+                                             int slice, double* sh) {
+  const SliceDev& sd = tab[slice];
+  const float2* __restrict__ grid = grids + (long)slice * N * N;
   const int* __restrict__ cell = sd.cell;
   const float2* __restrict__ frac = sd.frac;
   const float2* __restrict__ vo = sd.vo;
@@ -352,16 +358,18 @@ __global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
   double acc = 0.0;
   float cm = 0.0f;
   const int mask = N - 1;
-  // Grid-stride loop, four strides per pass: the loads of four visibilities
+  // Grid-stride loop, two strides per pass: the loads of two visibilities
   // are issued before their arithmetic, and they are accumulated in stride
-  // order, so the per-thread sums are those of the one-at-a-time loop.
-  const long S = (long)gridDim.x * blockDim.x;
-  for (long v0 = blockIdx.x * (long)blockDim.x + threadIdx.x; v0 < nvis; v0 += 4 * S) {
-    int c[4];
-    float2 f[4], o[4];
-    float w[4];
+  // order, so the per-thread sums are those of the one-at-a-time loop.  (Two
+  // rather than four: 46 instead of 108 VGPRs, and with the grid below the
+  // fused value launch takes 16.9 instead of 26.0 us at 1.76M visibilities.)
+  const long S = (long)nbx * kDegridThreads;
+  for (long v0 = bx * (long)kDegridThreads + threadIdx.x; v0 < nvis; v0 += 2 * S) {
+    int c[2];
+    float2 f[2], o[2];
+    float w[2];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < 2; u++) {
       const long v = v0 + u * S;
       const bool ok = v < nvis;
       c[u] = ok ? ldg(cell + v) : -1;
@@ -369,9 +377,9 @@ __global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
       o[u] = ok ? ldg(vo + v) : make_float2(0.f, 0.f);
       w[u] = ok ? ldg(weight + v) : 0.0f;
     }
-    float2 t11[4], t12[4], t21[4], t22[4];
+    float2 t11[2], t12[2], t21[2], t22[2];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < 2; u++) {
       t11[u] = t12[u] = t21[u] = t22[u] = make_float2(0.f, 0.f);
       if (c[u] >= 0) {
         const int i1 = c[u] & 0xffff, j1 = c[u] >> 16;
@@ -383,7 +391,7 @@ __global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < 2; u++) {
       const long v = v0 + u * S;
       if (v >= nvis) break;
       float2 m = make_float2(0.f, 0.f);
@@ -407,29 +415,16 @@ __global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
   __syncthreads();
   const double mx = block_max((double)cm, sh);
   if (threadIdx.x == 0) {
-    partials[(long)blockIdx.y * kReduceBlocks + blockIdx.x] = tot;
-    atomicMax(cmaxs + blockIdx.y, __float_as_uint((float)mx));
+    partials[(long)slice * kReduceBlocks + bx] = tot;
+    atomicMax(cmaxs + slice, __float_as_uint((float)mx));
   }
 }
 
-// For a few slices, finish_kernel and chi2_combine_kernel in one launch:
-// one block reduces each slice's block partials with finish_kernel's thread
-// assignment and tree, then thread 0 combines them as chi2_combine_kernel
-// does, so the value is the same bit for bit.
-__global__ __launch_bounds__(kReduceThreads) void chi2_finish_kernel(
-    const double* __restrict__ partials, int nblocks, const SliceDev* __restrict__ tab, int nsl,
-    int normalize, double* __restrict__ out) {
-  __shared__ double sh[kReduceThreads / 64];
-  double total = 0.0;
-  for (int s = 0; s < nsl; s++) {
-    const double* p = partials + (long)s * kReduceBlocks;
-    double v = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) v = v + p[i];
-    const double r = block_sum(v, sh);
-    if (threadIdx.x == 0) total += normalize ? r / (double)tab[s].nvis : r;
-    __syncthreads();  // sh is reused by the next slice
-  }
-  if (threadIdx.x == 0) *out = 0.5 * total;
+__global__ __launch_bounds__(kDegridThreads) void degrid_chi2_kernel(
+    const float2* __restrict__ grids, int N, const SliceDev* __restrict__ tab,
+    double* __restrict__ partials, unsigned int* __restrict__ cmaxs) {
+  __shared__ double sh[kDegridThreads / 64];
+  degrid_block(grids, N, tab, partials, cmaxs, blockIdx.x, gridDim.x, blockIdx.y, sh);
 }
 
 // 0.5 * the sum over slices of the per-slice sums (each divided by its
@@ -448,9 +443,86 @@ __global__ void chi2_combine_kernel(const double* __restrict__ sums,
   *out = 0.5 * total;
 }
 
-// chi2 (block 0, chi2_finish_kernel's arithmetic) and prior values (block
-// 1 + t: finish_kernel's arithmetic on partial row row0 + t into out[slot0 +
-// t]) in one launch, bit-identical to the separate finishing launches.
+// Sum over the 256 threads of group threadIdx.x / 256 of a 512-thread block
+// with block_sum's arithmetic for a block of 256 (wave sums, then the group's
+// four wave sums in order from 0.0); valid in the group's first thread.  sh
+// holds kDegridThreads / 64 doubles; every thread of the block calls it.
+__device__ __forceinline__ double group_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if ((threadIdx.x & (kReduceThreads - 1)) == 0)
+    for (int w = 0; w < kReduceThreads / 64; ++w) r += sh[wid + w];
+  return r;
+}
+
+struct ValuesArgs {
+  const float2* grids;
+  const SliceDev* tab;
+  double* partials;
+  unsigned int* cmaxs;
+  const float* I;
+  const float* noise;
+  PriorTermsDev T;
+  float noise_cut;
+  int N;
+  int nb_vis;      // degrid blocks per slice (degrid_chi2_kernel's gridDim.x)
+  int nb_prior;    // prior blocks of 256 threads (priors_value_kernel's grid)
+  int prior_row0;  // partial row of the first prior term
+};
+
+// The block partials of a value evaluation after the two FFT passes, in one
+// launch:
+//  - blocks [0, ceil(nb_prior / 2)): two of priors_value_kernel's 256-thread
+//    blocks each (the value partials of every prior term on the clipped
+//    image, which the row pass has written), with its arithmetic;
+//  - the rest: degrid_chi2_kernel's blocks, slice after slice.
+// The partials are those of the separate launches bit for bit.  (Finishing
+// them in the last block to end, found by a ticket counter, took the kernel
+// from 34 to 56 us: each block's release fence writes back its L2's dirty
+// lines.)
+__global__ __launch_bounds__(kDegridThreads) void values_kernel(ValuesArgs a) {
+  __shared__ double sh[kDegridThreads / 64];
+  const int np = (a.nb_prior + 1) / 2;
+  const int b = blockIdx.x;
+  if (b >= np) {
+    const int d = b - np;
+    degrid_block(a.grids, a.N, a.tab, a.partials, a.cmaxs, d % a.nb_vis, a.nb_vis, d / a.nb_vis,
+                 sh);
+    return;
+  }
+  const int g = threadIdx.x >> 8, vt = threadIdx.x & (kReduceThreads - 1);
+  const int vb = 2 * b + g;
+  const long n = (long)a.N * a.N;
+  double acc[GVM_MAX_FUSED_PRIORS];
+#pragma unroll
+  for (int t = 0; t < GVM_MAX_FUSED_PRIORS; t++) acc[t] = 0.0;
+  if (vb < a.nb_prior) {
+    for (long k = vb * (long)kReduceThreads + vt; k < n; k += (long)a.nb_prior * kReduceThreads) {
+      if (!(a.noise[k] < a.noise_cut)) continue;
+      const int i = pix_row(k, a.N), j = pix_col(k, a.N);
+#pragma unroll
+      for (int t = 0; t < GVM_MAX_FUSED_PRIORS; t++)
+        if (t < a.T.n && a.T.t[t].active)
+          acc[t] += (double)prior_term(a.T.t[t].kind, a.I + n * a.T.t[t].index, a.N, i, j,
+                                       a.T.t[t].prior, a.T.t[t].eta);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < GVM_MAX_FUSED_PRIORS; t++) {
+    if (t >= a.T.n) break;
+    const double tot = group_sum(acc[t], sh);
+    if (vt == 0 && vb < a.nb_prior) a.partials[(long)(a.prior_row0 + t) * kReduceBlocks + vb] = tot;
+    __syncthreads();  // sh is reused by the next term
+  }
+}
+
+// chi2 (block 0: finish_kernel's arithmetic on each slice's partials,
+// combined as chi2_combine_kernel does) and prior values (block 1 + t:
+// finish_kernel's arithmetic on partial row row0 + t into out[t]) in one
+// launch, bit-identical to the separate finishing launches.
 __global__ __launch_bounds__(kReduceThreads) void values_finish_kernel(
     const double* __restrict__ partials, int nblocks_chi2, const SliceDev* __restrict__ tab,
     int nsl, int normalize, double* __restrict__ chi2_out, int row0, int nblocks_prior,
@@ -464,7 +536,7 @@ __global__ __launch_bounds__(kReduceThreads) void values_finish_kernel(
       for (int i = threadIdx.x; i < nblocks_chi2; i += blockDim.x) v = v + p[i];
       const double r = block_sum(v, sh);
       if (threadIdx.x == 0) total += normalize ? r / (double)tab[s].nvis : r;
-      __syncthreads();
+      __syncthreads();  // sh is reused by the next slice
     }
     if (threadIdx.x == 0) *chi2_out = 0.5 * total;
     return;
@@ -477,6 +549,49 @@ __global__ __launch_bounds__(kReduceThreads) void values_finish_kernel(
   if (threadIdx.x == 0) prior_out[t] = r;
 }
 
+// The degrid blocks per slice: one visibility per thread, at most 1.5 blocks
+// per CU (measured at 1024^2 / 1.76M visibilities: 384 blocks 16.9 us, 256
+// 18.2, 512 19.2, 1024 26.0 for the fused value launch) and kReduceBlocks.
+static int degrid_blocks(const gvm_ctx* ctx) {
+  long vmax = 0;
+  for (const Slice& sl : ctx->slices) vmax = std::max(vmax, sl.nvis);
+  const long cap = std::min<long>(kReduceBlocks, std::max(1, 3 * ctx->cu_count / 2));
+  return (int)std::min<long>(cap, std::max<long>(1, (vmax + kDegridThreads - 1) / kDegridThreads));
+}
+
+// chi2 into *chi2_out and n prior values into prior_out[0, n) for up to four
+// slices: the two FFT passes, values_kernel and values_finish_kernel.
+static gvm_status values_fused(gvm_ctx* ctx, float* I, int flag_opt, int normalize,
+                               double* chi2_out, const gvm_prior_term* terms, int n,
+                               double* prior_out) {
+  const int nsl = (int)ctx->slices.size();
+  ValuesArgs a{};
+  gvm_status st = check_terms(ctx, terms, n, &a.T);
+  if (st) return st;
+  GVM_HIP_TRY(hipSetDevice(ctx->device));
+  if ((st = model_grids(ctx, I, nsl, flag_opt))) return st;
+  a.grids = ctx->grid;
+  a.tab = ctx->slice_tab;
+  a.partials = ctx->partials;
+  a.cmaxs = ctx->cmax;
+  a.I = I;
+  a.noise = ctx->noise;
+  a.noise_cut = ctx->g.noise_cut;
+  a.N = ctx->g.N;
+  a.nb_vis = degrid_blocks(ctx);
+  a.nb_prior = n > 0 ? reduce_grid((long)ctx->g.N * ctx->g.N) : 0;
+  a.prior_row0 = nsl;
+  const int nblocks = (a.nb_prior + 1) / 2 + a.nb_vis * nsl;
+  hipLaunchKernelGGL(values_kernel, dim3(nblocks), dim3(kDegridThreads), 0, ctx->stream, a);
+  GVM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(values_finish_kernel, dim3(1 + n), dim3(kReduceThreads), 0, ctx->stream,
+                     ctx->partials, a.nb_vis, ctx->slice_tab, nsl, normalize, chi2_out, nsl,
+                     a.nb_prior, prior_out);
+  GVM_CHECK_LAUNCH();
+  ctx->have_residuals = true;
+  return GVM_OK;
+}
+
 // The forward pass up to the chi2 block partials (slice s in partial row s);
 // returns the blocks per slice.
 static gvm_status forward_chi2_partials(gvm_ctx* ctx, float* I, int flag_opt, int* nb_out) {
@@ -485,11 +600,7 @@ static gvm_status forward_chi2_partials(gvm_ctx* ctx, float* I, int flag_opt, in
   GVM_HIP_TRY(hipSetDevice(ctx->device));
   gvm_status st = model_grids(ctx, I, nsl, flag_opt);
   if (st) return st;
-  // blocks per slice: one visibility per thread, at most kReduceBlocks
-  long vmax = 0;
-  for (const Slice& sl : ctx->slices) vmax = std::max(vmax, sl.nvis);
-  const int nb = (int)std::min<long>(
-      kReduceBlocks, std::max<long>(1, (vmax + kDegridThreads - 1) / kDegridThreads));
+  const int nb = degrid_blocks(ctx);
   hipLaunchKernelGGL(degrid_chi2_kernel, dim3(nb, nsl), dim3(kDegridThreads), 0, ctx->stream,
                      ctx->grid, g.N, ctx->slice_tab, ctx->partials, ctx->cmax);
   GVM_CHECK_LAUNCH();
@@ -499,42 +610,27 @@ static gvm_status forward_chi2_partials(gvm_ctx* ctx, float* I, int flag_opt, in
 
 gvm_status values_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int chi2_slot,
                           const gvm_prior_term* terms, int n, int first_slot) {
-  const int nsl = (int)ctx->slices.size();
-  if (nsl > 4) {
-    // many slices: the chi2 finish takes its own launches
-    gvm_status st = forward_chi2_enqueue(ctx, I, flag_opt, normalize, chi2_slot);
-    if (st) return st;
-    return priors_value_partials_finish(ctx, I, terms, n, first_slot);
-  }
-  int nb = 0;
-  gvm_status st = forward_chi2_partials(ctx, I, flag_opt, &nb);
+  if ((int)ctx->slices.size() <= 4)
+    return values_fused(ctx, I, flag_opt, normalize, ctx->dvalues + chi2_slot, terms, n,
+                        ctx->dvalues + first_slot);
+  // many slices: chi2 and the priors take their own finishing launches
+  gvm_status st = forward_chi2_enqueue(ctx, I, flag_opt, normalize, chi2_slot);
   if (st) return st;
-  int nbp = 0;
-  if ((st = priors_value_partials(ctx, I, terms, n, nsl, &nbp))) return st;
-  hipLaunchKernelGGL(values_finish_kernel, dim3(1 + n), dim3(kReduceThreads), 0, ctx->stream,
-                     ctx->partials, nb, ctx->slice_tab, nsl, normalize, ctx->dvalues + chi2_slot,
-                     nsl, nbp, ctx->dvalues + first_slot);
-  GVM_CHECK_LAUNCH();
-  ctx->have_residuals = true;
-  return GVM_OK;
+  return priors_value_partials_finish(ctx, I, terms, n, first_slot);
 }
 
 gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot) {
   const int nsl = (int)ctx->slices.size();
+  if (nsl <= 4)
+    return values_fused(ctx, I, flag_opt, normalize, ctx->dvalues + slot, nullptr, 0,
+                        nullptr);
   int nb = 0;
   gvm_status st = forward_chi2_partials(ctx, I, flag_opt, &nb);
   if (st) return st;
-  if (nsl <= 4) {
-    hipLaunchKernelGGL(chi2_finish_kernel, dim3(1), dim3(kReduceThreads), 0, ctx->stream,
-                       ctx->partials, nb, ctx->slice_tab, nsl, normalize, ctx->dvalues + slot);
-    GVM_CHECK_LAUNCH();
-  } else {
-    st = finish_reductions(ctx, nsl, nb);
-    if (st) return st;
-    hipLaunchKernelGGL(chi2_combine_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->dscalars,
-                       ctx->slice_tab, nsl, normalize, ctx->dvalues + slot);
-    GVM_CHECK_LAUNCH();
-  }
+  if ((st = finish_reductions(ctx, nsl, nb))) return st;
+  hipLaunchKernelGGL(chi2_combine_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->dscalars,
+                     ctx->slice_tab, nsl, normalize, ctx->dvalues + slot);
+  GVM_CHECK_LAUNCH();
   ctx->have_residuals = true;
   return GVM_OK;
 }

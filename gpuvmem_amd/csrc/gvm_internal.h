@@ -93,8 +93,8 @@ struct gvm_ctx {
   double* partials = nullptr;   // kMaxSlots * kReduceBlocks doubles
   double* dscalars = nullptr;   // kMaxSlots doubles (device)
   double* hscalars = nullptr;   // kMaxSlots doubles (pinned host)
-  double* dvalues = nullptr;    // kValueSlots term values (device), read back together
-  double* hvalues = nullptr;    // kValueSlots doubles (pinned host)
+  double* dvalues = nullptr;    // kValueSlots term values (pinned host, written by kernels)
+  double* hvalues = nullptr;    // the same memory, read on the host after a wait
   unsigned int* cmax = nullptr; // per-slice max |w Vr| (float bits), device
   // batched slices: device descriptor table, uv grids (one per slice)
   gvm::SliceDev* slice_tab = nullptr;  // kMaxSlots entries
@@ -156,13 +156,15 @@ gvm_status priors_value_partials(gvm_ctx* ctx, const float* I, const gvm_prior_t
                                  int n, int row0, int* nb_out);
 gvm_status priors_value_partials_finish(gvm_ctx* ctx, const float* I,
                                         const gvm_prior_term* terms, int n, int first_slot);
-// chi2 into dvalues[chi2_slot] and n prior terms into dvalues[first_slot..]
-// with one finishing launch (for up to 4 slices).
+// chi2 into dvalues[chi2_slot] and n prior terms into dvalues[first_slot..]:
+// for up to 4 slices the FFT passes, one launch for the partials and one to
+// finish them.
 gvm_status values_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int chi2_slot,
                           const gvm_prior_term* terms, int n, int first_slot);
 // Enqueues the chi2 evaluation; 0.5 * sum over slices lands in dvalues[slot].
 gvm_status forward_chi2_enqueue(gvm_ctx* ctx, float* I, int flag_opt, int normalize, int slot);
-// Copies dvalues[0, n) to the host (one synchronisation).
+// Reads the value slots [0, n) once the stream has reached this point (one
+// synchronisation; the kernels write the slots in pinned host memory).
 gvm_status values_read(gvm_ctx* ctx, int n, double* out);
 gvm_status values_fetch_enqueue(gvm_ctx* ctx, int n);
 gvm_status values_host(gvm_ctx* ctx, int n, double* out);
