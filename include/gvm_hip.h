@@ -222,9 +222,10 @@ gvm_status gvm_chi2(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize,
 
 /* Asynchronous value forms, with no counterpart in the reference: they
  * enqueue an evaluation on the context's stream and leave its FP64 value in
- * device value slot `slot` (0..15) instead of returning it, so a caller can
+ * value slot `slot` (0..15) instead of returning it, so a caller can
  * evaluate several terms of an objective with one synchronisation
- * (gvm_values_read).  Results are identical to the synchronous forms.  A slot
+ * (gvm_values_read).  The slots are pinned host memory that the kernels
+ * write directly.  Results are identical to the synchronous forms.  A slot
  * keeps its value until it is written again. */
 #define GVM_VALUE_SLOTS 16
 gvm_status gvm_chi2_enqueue(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize, int slot);
@@ -234,8 +235,10 @@ gvm_status gvm_tsv_value_enqueue(gvm_ctx* ctx, const float* I_dev, int index, in
 gvm_status gvm_laplacian_value_enqueue(gvm_ctx* ctx, const float* I_dev, int index, int slot);
 /* Waits for the stream and copies value slots [0, n) to out. */
 gvm_status gvm_values_read(gvm_ctx* ctx, int n, double* out);
-/* gvm_values_read in two halves: enqueue the copy of slots [0, n) to the
- * context's pinned host buffer, then wait for the stream and read it. */
+/* gvm_values_read in two halves: the first enqueues nothing (the kernels
+ * write the slots in host memory themselves; it is kept so that callers and
+ * captured graphs written for a copy still work), the second waits for the
+ * stream and reads slots [0, n). */
 gvm_status gvm_values_fetch_enqueue(gvm_ctx* ctx, int n);
 gvm_status gvm_values_host(gvm_ctx* ctx, int n, double* out);
 
@@ -341,9 +344,9 @@ typedef struct {
 gvm_status gvm_priors_value_enqueue(gvm_ctx* ctx, const float* I_dev, const gvm_prior_term* terms,
                                     int n, int first_slot);
 /* gvm_chi2_enqueue into chi2_slot followed by gvm_priors_value_enqueue into
- * [first_slot, first_slot + n), with one finishing launch for all the
- * values (for up to 4 slices; more take chi2's own finishing launches).
- * Same values bit for bit. */
+ * [first_slot, first_slot + n): for up to 4 slices the prior partials share
+ * the degrid's launch and one finishing launch writes all the values (more
+ * slices take chi2's own finishing launches).  Same values bit for bit. */
 gvm_status gvm_values_enqueue(gvm_ctx* ctx, float* I_dev, int flag_opt, int normalize,
                               int chi2_slot, const gvm_prior_term* terms, int n, int first_slot);
 /* ObjectiveFunction's gradient assembly after Chi2::calcGi in one pass:
