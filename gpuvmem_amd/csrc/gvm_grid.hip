@@ -12,7 +12,8 @@
 // inside each cell, and one thread per cell then adds that cell's entries in
 // sequence.  The result is the reference's single-thread sum, bit for bit,
 // and does not depend on scheduling.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -73,7 +74,7 @@ int key_bits(uint32_t sentinel) {
 struct CellSort {
   DevMem k0, k1, v0, v1, tmp;
   size_t cap = 0, tmp_bytes = 0;
-  hipcub::DoubleBuffer<uint32_t> kb, vb;
+  rocprim::double_buffer<uint32_t> kb, vb;
 
   gvm_status reserve(long n, const char* what) {
     if ((size_t)n <= cap) return GVM_OK;
@@ -84,10 +85,10 @@ struct CellSort {
         (st = dev_alloc(v1, sizeof(uint32_t) * n, what)))
       return st;
     cap = (size_t)n;
-    hipcub::DoubleBuffer<uint32_t> k(k0.as<uint32_t>(), k1.as<uint32_t>());
-    hipcub::DoubleBuffer<uint32_t> v(v0.as<uint32_t>(), v1.as<uint32_t>());
+    rocprim::double_buffer<uint32_t> k(k0.as<uint32_t>(), k1.as<uint32_t>());
+    rocprim::double_buffer<uint32_t> v(v0.as<uint32_t>(), v1.as<uint32_t>());
     size_t bytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)n, 0, 32) != hipSuccess) {
+    if (rocprim::radix_sort_pairs(nullptr, bytes, k, v, (size_t)n, 0u, 32u) != hipSuccess) {
       set_error(std::string(what) + ": radix sort sizing failed");
       return GVM_ERR_HIP;
     }
@@ -97,10 +98,11 @@ struct CellSort {
   uint32_t* in_keys() const { return k0.as<uint32_t>(); }
   uint32_t* in_ids() const { return v0.as<uint32_t>(); }
   gvm_status sort(long n, int bits, const char* what) {
-    kb = hipcub::DoubleBuffer<uint32_t>(k0.as<uint32_t>(), k1.as<uint32_t>());
-    vb = hipcub::DoubleBuffer<uint32_t>(v0.as<uint32_t>(), v1.as<uint32_t>());
+    kb = rocprim::double_buffer<uint32_t>(k0.as<uint32_t>(), k1.as<uint32_t>());
+    vb = rocprim::double_buffer<uint32_t>(v0.as<uint32_t>(), v1.as<uint32_t>());
     size_t bytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kb, vb, (int)n, 0, bits) != hipSuccess) {
+    if (rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, (size_t)n, 0u, (unsigned)bits) !=
+        hipSuccess) {
       set_error(std::string(what) + ": radix sort sizing failed");
       return GVM_ERR_HIP;
     }
@@ -110,14 +112,15 @@ struct CellSort {
       tmp_bytes = bytes;
     }
     bytes = tmp_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kb, vb, (int)n, 0, bits) != hipSuccess) {
+    if (rocprim::radix_sort_pairs(tmp.p, bytes, kb, vb, (size_t)n, 0u, (unsigned)bits) !=
+        hipSuccess) {
       set_error(std::string(what) + ": radix sort failed");
       return GVM_ERR_HIP;
     }
     return GVM_OK;
   }
-  const uint32_t* keys() { return kb.Current(); }
-  const uint32_t* ids() { return vb.Current(); }
+  const uint32_t* keys() { return kb.current(); }
+  const uint32_t* ids() { return vb.current(); }
 };
 
 // ---- convolutional gridding -----------------------------------------------
@@ -515,12 +518,12 @@ extern "C" gvm_status gvm_grid_visibilities_dev(gvm_ctx* ctx, const gvm_ckernel*
                      n, flag.as<int>());
   if ((st = launch_ok(what))) return st;
   size_t sbytes = 0;
-  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, sbytes, flag.as<int>(), rank.as<int>(),
-                                               (int)n));
+  GVM_HIP_TRY(rocprim::exclusive_scan(nullptr, sbytes, flag.as<int>(), rank.as<int>(), 0,
+                                      (size_t)n, rocprim::plus<int>()));
   DevMem stmp;
   if ((st = dev_alloc(stmp, sbytes, what))) return st;
-  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(stmp.p, sbytes, flag.as<int>(), rank.as<int>(),
-                                               (int)n));
+  GVM_HIP_TRY(rocprim::exclusive_scan(stmp.p, sbytes, flag.as<int>(), rank.as<int>(), 0,
+                                      (size_t)n, rocprim::plus<int>()));
   int last[2] = {0, 0};
   GVM_HIP_TRY(hipMemcpy(&last[0], rank.as<int>() + n - 1, sizeof(int), hipMemcpyDeviceToHost));
   GVM_HIP_TRY(hipMemcpy(&last[1], flag.as<int>() + n - 1, sizeof(int), hipMemcpyDeviceToHost));
@@ -539,12 +542,12 @@ extern "C" gvm_status gvm_grid_visibilities_dev(gvm_ctx* ctx, const gvm_ckernel*
   if ((st = launch_ok(what))) return st;
   // exclusive prefix sum of the keep flags: output row of each kept cell
   size_t kbytes = 0;
-  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, kbytes, keep.as<int>(), pos.as<int>(),
-                                               (int)nruns));
+  GVM_HIP_TRY(rocprim::exclusive_scan(nullptr, kbytes, keep.as<int>(), pos.as<int>(), 0,
+                                      (size_t)nruns, rocprim::plus<int>()));
   if (kbytes > sbytes && (st = dev_alloc(stmp, kbytes, what))) return st;
   kbytes = std::max(kbytes, sbytes);
-  GVM_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(stmp.p, kbytes, keep.as<int>(), pos.as<int>(),
-                                               (int)nruns));
+  GVM_HIP_TRY(rocprim::exclusive_scan(stmp.p, kbytes, keep.as<int>(), pos.as<int>(), 0,
+                                      (size_t)nruns, rocprim::plus<int>()));
   GVM_HIP_TRY(hipMemcpy(&last[0], pos.as<int>() + nruns - 1, sizeof(int), hipMemcpyDeviceToHost));
   GVM_HIP_TRY(hipMemcpy(&last[1], keep.as<int>() + nruns - 1, sizeof(int), hipMemcpyDeviceToHost));
   const long count = (long)last[0] + last[1];
