@@ -4,10 +4,13 @@ adjoint GEMM over a table of visibility ranges of all slices, one epilogue).
 Checked against the FP64 oracle on shrunken versions of the multi-frequency
 workload (uneven per-channel counts from flagging, spectral indices), for
 the chain rule to I_nu_0 and to alpha, and with many slices."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
 import oracle
+from gpuvmem_amd import native
 from gpuvmem_amd.device import Context
 from gpuvmem_amd.problem import make_mfs_problem
 
@@ -76,3 +79,48 @@ def test_mfs_many_slices_and_normalize():
         err = _check(p, orc, "f16x3", img, flag, normalize=(flag == 1))
         print(f"flag {flag}: gradient rel L2 {err:.3e}")
         assert err < GRAD_TOL
+
+
+class _PriorTerm(C.Structure):
+    """gvm_prior_term of include/gvm_hip.h."""
+    _fields_ = [("kind", C.c_int), ("index", C.c_int), ("image_to_add", C.c_int),
+                ("active", C.c_int), ("prior", C.c_float), ("eta", C.c_float),
+                ("lambda_", C.c_float), ("pad", C.c_float)]
+
+
+@pytest.mark.parametrize("n_spw,flag_opt,normalize", [(1, 0, False), (1, 1, True), (2, 2, False)])
+def test_value_launch_is_bitwise_the_separate_launches(n_spw, flag_opt, normalize):
+    """gvm_values_enqueue (chi2 and three priors; for up to four slices the
+    degrid and the prior partials share one launch and one launch finishes
+    them, for more slices each takes its own) leaves the same values bit for
+    bit as gvm_chi2_enqueue followed by the one-term prior launches, for 3
+    slices and for 6; chi2 also matches the FP64 oracle."""
+    p = _small_mfs(n_spw=n_spw, n_chan=3, npix=64, n_antennas=8)
+    nsl = len(p.slices)
+    print(f"case: value launch {nsl} slices {p.M}^2 flag_opt {flag_opt} normalize {normalize}",
+          flush=True)
+    img = _image(p, 7)
+    terms = (_PriorTerm * 3)()
+    for t, (kind, index, prior, eta) in enumerate([(0, 0, 0.001, -1.0), (1, 1, 0.0, 0.0),
+                                                    (2, 0, 0.0, 0.0)]):
+        terms[t].kind, terms[t].index, terms[t].image_to_add = kind, index, index
+        terms[t].active, terms[t].prior, terms[t].eta, terms[t].lambda_ = 1, prior, eta, 1.0
+    ctx = Context(p)
+    lib = ctx.lib
+    d = ctx.image(img)
+    native.check(lib.gvm_values_enqueue(ctx.ctx, d.ptr, flag_opt, int(normalize), 0,
+                                        C.cast(terms, C.c_void_p), 3, 1), "values")
+    fused = (C.c_double * 4)()
+    native.check(lib.gvm_values_read(ctx.ctx, 4, fused), "read")
+    d2 = ctx.image(img)
+    native.check(lib.gvm_chi2_enqueue(ctx.ctx, d2.ptr, flag_opt, int(normalize), 4), "chi2")
+    native.check(lib.gvm_entropy_value_enqueue(ctx.ctx, d2.ptr, 0, 0.001, -1.0, 5), "entropy")
+    native.check(lib.gvm_tsv_value_enqueue(ctx.ctx, d2.ptr, 1, 6), "tsv")
+    native.check(lib.gvm_laplacian_value_enqueue(ctx.ctx, d2.ptr, 0, 7), "laplacian")
+    sep = (C.c_double * 8)()
+    native.check(lib.gvm_values_read(ctx.ctx, 8, sep), "read")
+    ctx.close()
+    assert list(fused) == list(sep)[4:8], (list(fused), list(sep)[4:8])
+    ref, _, _ = oracle.Oracle(p).chi2(img, flag_opt=flag_opt, normalize=normalize)
+    assert abs(fused[0] - ref) <= CHI2_TOL * abs(ref), (fused[0], ref)
+    assert all(v != 0.0 for v in fused[1:])
