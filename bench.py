@@ -576,7 +576,7 @@ def main() -> int:
                          "multi-frequency 512^2 I_nu_0 + alpha, 4 spw x 64 channels, ~61M vis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--tti-observations", type=int, default=3,
+    ap.add_argument("--tti-observations", type=int, default=5,
                     help="time-to-image: observations (sky and noise seeds) to take the median over")
     ap.add_argument("--tti-iterations", type=int, default=100,
                     help="time-to-image leg of the default run: L-BFGS iteration cap per form")
