@@ -294,6 +294,7 @@ extern "C" gvm_status gvm_destroy(gvm_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& s : ctx->slices) free_slice(s);
   (void)hipFree(ctx->noise);
+  (void)hipFree(ctx->mask_bits);
   if (ctx->gcf) (void)hipFree(ctx->gcf);
   (void)hipFree(ctx->grid);
   (void)hipFree(ctx->twiddle);
@@ -340,6 +341,12 @@ static gvm_status create_impl(const gvm_geometry* g, const float* noise_host, gv
   std::vector<float> noise(noise_host, noise_host + n);
   gvm_status st = upload(&ctx->noise, noise);
   if (st) return st;
+  // the mask (noise < noise_cut) as bits, pixel k in bit k % 32 of word k / 32:
+  // the adjoint writes its partial tiles on the mask only
+  std::vector<unsigned int> bits((size_t)((n + 31) / 32), 0u);
+  for (long k = 0; k < n; k++)
+    if (noise[k] < g->noise_cut) bits[k >> 5] |= 1u << (k & 31);
+  if ((st = upload(&ctx->mask_bits, bits))) return st;
   if ((st = dev_alloc(&ctx->grid, n))) return st;
   ctx->grid_slices = 1;
   if ((st = dev_alloc(&ctx->slice_tab, kMaxSlots))) return st;

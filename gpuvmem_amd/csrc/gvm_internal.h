@@ -82,6 +82,7 @@ struct gvm_ctx {
   hipStream_t stream = nullptr;
   // image-space tables
   float* noise = nullptr;       // float[M*N]
+  unsigned int* mask_bits = nullptr;  // noise < noise_cut, one bit per pixel
   float* gcf = nullptr;         // gridding correction image float[M*N], or none
   float2* grid = nullptr;       // uv grid, complex float[N*M]
   float2* twiddle = nullptr;    // e^{+2 pi i k / N}, k < N
