@@ -446,16 +446,19 @@ __global__ __launch_bounds__(256) void polar_kernel(const SliceDev* __restrict__
 
 // Direct sum (the reference's algorithm, exact w term), one thread per pixel,
 // visibilities staged through LDS.  FP32 terms, FP64 accumulation per 256
-// visibilities.  Writes G to part[0].
+// visibilities.  Writes G to part[0] on the mask; a block with no pixel on
+// the mask returns at once (the reference's DChi2 skips those pixels too).
 __global__ __launch_bounds__(256) void adjoint_direct_kernel(
     const float4* __restrict__ phs, const float* __restrict__ wcoord,
     const float* __restrict__ weight, const float2* __restrict__ vr, long nvis,
     const float2* __restrict__ xtab, const float2* __restrict__ ytab, int N, double dx_rad,
-    double dy_rad, float* __restrict__ part) {
+    double dy_rad, const unsigned int* __restrict__ mbits, float* __restrict__ part) {
   __shared__ float4 sp[256];
   __shared__ float4 sc[256];  // (w, cr, ci, 0)
   const long pix = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const bool valid = pix < (long)N * N;
+  const bool on_mask = valid && ((mbits[pix >> 5] >> (pix & 31)) & 1u);
+  if (!__syncthreads_or(on_mask)) return;
   const int i = valid ? (int)(pix / N) : 0, j = valid ? (int)(pix % N) : 0;
   const float2 xt = xtab[j], yt = ytab[i];
   const double x = xt.x * dx_rad, y = yt.x * dy_rad;
@@ -486,7 +489,7 @@ __global__ __launch_bounds__(256) void adjoint_direct_kernel(
     }
     acc += (double)part_acc;
   }
-  if (valid) part[pix] = (float)acc;
+  if (on_mask) part[pix] = (float)acc;
 }
 
 // Per pixel inside the mask and per slice in slice order: the FP64 sum of
@@ -689,7 +692,7 @@ gvm_status adjoint(gvm_ctx* ctx, const float* I, float* result, int flag_opt, in
       const Slice& s = ctx->slices[si];
       hipLaunchKernelGGL(adjoint_direct_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                          ctx->stream, s.phs, s.wcoord, s.weight, s.vr, s.nvis, ctx->xtab,
-                         ctx->ytab, N, dxr, dyr, ctx->adj_part + (long)si * n);
+                         ctx->ytab, N, dxr, dyr, ctx->mask_bits, ctx->adj_part + (long)si * n);
       GVM_CHECK_LAUNCH();
     }
   } else {
