@@ -829,8 +829,12 @@ extern "C" gvm_status gvm_autotune_adjoint(gvm_ctx* ctx, const long* candidates,
   GVM_REQUIRE(ctx->adjoint_mode != GVM_ADJOINT_DIRECT,
               "gvm_autotune_adjoint: the direct form has no range length");
   if (!candidates) {
+    // the default list up to the longest range this image size allows
     candidates = kDefault;
-    ncand = (int)(sizeof kDefault / sizeof kDefault[0]);
+    ncand = 0;
+    while (ncand < (int)(sizeof kDefault / sizeof kDefault[0]) &&
+           kDefault[ncand] <= max_tuned_split_vis(ctx->g.N))
+      ncand++;
   }
   GVM_REQUIRE(ncand >= 1 && ncand <= 64, "gvm_autotune_adjoint: 1 to 64 candidates");
   for (int c = 0; c < ncand; c++)

@@ -334,6 +334,7 @@ static gvm_status create_impl(const gvm_geometry* g, const float* noise_host, gv
     return GVM_ERR_NODEV;
   }
   ctx->cu_count = prop.multiProcessorCount;
+  ctx->split_vis = default_split_vis(N);
   GVM_HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   for (hipEvent_t& e : ctx->tev) GVM_HIP_TRY(hipEventCreate(&e));
 

@@ -110,7 +110,7 @@ struct gvm_ctx {
   int nsplit_total = 0;
   int split_mode = -1;          // adjoint mode the table was built for (-1: stale)
   int adjoint_mode = GVM_ADJOINT_BF16X3;
-  long split_vis = 6144;        // target visibilities per adjoint range (K tile)
+  long split_vis = 6144;        // target visibilities per adjoint range (K tile), default_split_vis
   int fft_mode = 0;             // FFT batching: 0 automatic, 1 wide, 2 narrow
   std::vector<gvm::Slice> slices;
   bool have_residuals = false;
@@ -133,6 +133,14 @@ struct gvm_ctx {
 };
 
 namespace gvm {
+
+// The adjoint's default range length and the longest the autotuner tries, by
+// image size.  The FP32 accumulation error of a range grows with the square
+// root of its length; at 2048^2 and 4.35 M visibilities ranges of 6144 put
+// the gradient at 8e-6 (bf16x3) and 1.0e-5 (f16x3) relative L2 against FP64
+// sums on sampled pixels, ranges of 3072 at 5-6e-6.
+inline long default_split_vis(int N) { return N >= 2048 ? 3072 : 6144; }
+inline long max_tuned_split_vis(int N) { return N >= 2048 ? 3072 : 8192; }
 
 // Adds the durations of up to n of the oldest pending timed adjoint launches
 // to the totals (waiting for each); n < 0 for all.

@@ -212,8 +212,11 @@ class Context:
                                                    C.byref(chosen), ms, C.byref(cached)),
                      "gvm_autotune_adjoint")
         names = list(candidates) if candidates else [1536, 2048, 3072, 4096, 6144, 8192]
+        # the default list stops at the longest range the image size allows
+        # (3072 from 2048^2 on); the candidates not timed keep 0
+        timed = {n: t for n, t in zip(names, list(ms)) if t > 0.0}
         return {"split_vis": chosen.value, "cached": bool(cached.value),
-                "ms": None if cached.value else dict(zip(names, list(ms)))}
+                "ms": None if cached.value else timed}
 
     def adjoint_time_ms(self) -> tuple[float, int]:
         ms = C.c_double()

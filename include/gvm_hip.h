@@ -478,7 +478,9 @@ gvm_status gvm_timing_adjoint_ms(gvm_ctx* ctx, double* avg_ms, long* count);
  * adjoint accumulates one 256 x 256 output tile over a range of about
  * `split_vis` visibilities in FP32 (the ranges are summed in FP64): shorter
  * ranges keep the FP32 accumulation short, longer ones write fewer partial
- * tiles.  Default 6144; allowed 256 .. 16384. */
+ * tiles.  Default 6144, and 3072 for images of 2048^2 and larger (where
+ * ranges of 6144 bring the gradient's relative error near 1e-5); allowed
+ * 256 .. 16384. */
 gvm_status gvm_set_split_vis(gvm_ctx* ctx, long split_vis);
 /* Batching of the image-to-grid transform (no counterpart in the reference,
  * which calls cuFFT): 0 automatic (default), 1 wide blocks (many rows or
@@ -488,7 +490,8 @@ gvm_status gvm_set_fft_mode(gvm_ctx* ctx, int mode);
 gvm_status gvm_get_split_vis(const gvm_ctx* ctx, long* split_vis);
 /* Autotuner: times the adjoint of the current residuals (a gvm_chi2 must
  * have run) for each candidate range length (NULL: 1536, 2048, 3072, 4096,
- * 6144, 8192), with HIP events, and keeps the fastest.  The choice is cached
+ * 6144, 8192, up to 3072 for images of 2048^2 and larger), with HIP events,
+ * and keeps the fastest.  The choice is cached
  * per shape (M, total visibilities, slices, adjoint form, CU count) for the
  * process, and in the text file named by the GVM_TUNE_CACHE environment
  * variable when it is set; a cached shape is not timed again (*cached = 1).
