@@ -9,7 +9,9 @@ ObjectiveFunction::calcFunction + one calcGradient (the pair of
 src/frprmn.cu:130-135) at a fixed image, all inputs resident in HBM.
 
 Prints one JSON line.  The roofline figure is for the dominant kernel, the
-DChi2 adjoint GEMM: algorithmic work 4*M*N*Nvis flop per launch over its
+DChi2 adjoint GEMM: algorithmic work 4 flop per (pixel, visibility) over the
+pixels it computes (its 128 x 64 sub-tiles that hold a pixel on the mask; the
+reference's DChi2 computes the pixels on the mask) per launch, over its
 average duration (HIP events on the context's stream).  The cpu_baseline leg
 times the FP64 oracle on the host cores on a bounded sample (full-size
 forward chi2 + the separable FP64 gradient over a slice of the
@@ -726,7 +728,10 @@ def main() -> int:
     nvt = p.nvis_total
     nsl = len(p.all_slices())
     traffic, traffic_src = profiled_traffic(args.adjoint, nvt, ctx.split_vis())
-    flops = 4.0 * p.M * p.N * nvt
+    # the adjoint computes the sub-tiles that hold a pixel on the mask: the work
+    # of a launch is 4 flop per (computed pixel, visibility)
+    px = ctx.adjoint_computed_pixels()
+    flops = 4.0 * px * nvt
     achieved = flops / (adj_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.adjoint]
     line = {
@@ -768,6 +773,9 @@ def main() -> int:
             "kernel": f"adjoint_gemm_kernel ({args.adjoint})",
             "kernel_ms": adj_ms,
             "algorithmic_flop_per_launch": flops,
+            "computed_pixels": px,
+            "image_pixels": p.M * p.N,
+            "full_image_flop_per_launch": 4.0 * p.M * p.N * nvt,
         },
     }
     line["value_evaluation_us"] = value_latency(obj, d_img, ctx)

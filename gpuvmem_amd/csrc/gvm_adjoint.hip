@@ -90,7 +90,7 @@ struct AdjLds {
   unsigned char bl[2][kTile * kRowBytes];
   float4 rec_s[2][kKV];  // (a_hi, a_lo, b_hi, b_lo)
   float4 rec_d[2][kKV];  // (scale |c|, arg c in turns, w, 0)
-  unsigned int mask[kTile][kTile / 32];  // the output tile's mask bits
+  unsigned int mask[kTile / 32][kTile];  // the output tile's mask bits, word-major
 };
 
 template <>
@@ -101,7 +101,7 @@ struct AdjLds<PREC_F32> {
   unsigned char b[2][kTile * kRowBytes];
   float4 rec_s[2][kKV];
   float4 rec_d[2][kKV];
-  unsigned int mask[kTile][kTile / 32];
+  unsigned int mask[kTile / 32][kTile];
 };
 
 // Byte offset of 16-byte group g of a row, with an XOR swizzle of the group
@@ -296,6 +296,7 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
     }
   };
 
+  bool active = false;  // this wave's sub-tile has a pixel on the mask
   f32x16_t acc[4][2];
 #pragma unroll
   for (int a = 0; a < 4; a++)
@@ -372,7 +373,7 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int gw = tj * (kTile / 32) + mw + i;
-      sm.mask[mr][mw + i] = (gr < N && gw < N / 32) ? mbits[(long)gr * (N / 32) + gw] : 0u;
+      sm.mask[mw + i][mr] = (gr < N && gw < N / 32) ? mbits[(long)gr * (N / 32) + gw] : 0u;
     }
   }
 
@@ -386,6 +387,16 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
       load_rec(2);
     }
     __syncthreads();
+    // does this wave's 128 x 64 sub-tile hold any pixel on the mask?  (lane l
+    // looks at rows l and l + 64, both words of the sub-tile's columns)
+    {
+      bool any = false;
+#pragma unroll
+      for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int w2 = 0; w2 < 2; w2++) any |= sm.mask[(wc >> 5) + w2][wr + 64 * h + lane] != 0u;
+      active = __ballot(any) != 0ull;
+    }
     generate(0);
     __syncthreads();
     // main loop: one barrier per chunk
@@ -395,9 +406,11 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
       // before or after the multiply by the wave's group.  The multiply is one
       // instance outside the branches, so the accumulators keep one register
       // assignment through the loop (two instances, one per branch, cost
-      // about 4 % in copies between them).
+      // about 4 % in copies between them).  A wave whose sub-tile has no
+      // pixel on the mask skips its MFMAs (and writes nothing): at 1024^2
+      // with a disc-shaped mask 20 % of the sub-tiles, 9.6 % of the time.
       if (!group_b) generate(nxt);
-      multiply(cur);
+      if (active) multiply(cur);
       if (group_b) generate(nxt);
       if (wave == 0) {
         store_rec(cur);  // records of chunk c+2
@@ -412,18 +425,28 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
   //      layout, lane l, reg e: row 8*(e/4) + 4*(l/32) + e%4, column l%32 ----
   if (nchunks == 0) __syncthreads();  // the mask tile, when the loop has not ordered it
   float* out = part + (long)split * N * N;
+  // nothing of this wave's sub-tile is on the mask (a range with no
+  // visibilities, nchunks == 0, still writes its zeros: the epilogue reads it)
+  if (!active && nchunks > 0) return;
 #pragma unroll
   for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-    for (int cb = 0; cb < 2; cb++)
+    for (int cb = 0; cb < 2; cb++) {
+      // the mask words of this lane's 16 rows (4 groups of 4 consecutive rows),
+      // read before the stores
+      const int mw = (wc + cb * 32) >> 5, row0 = wr + rb * 32 + 4 * lh;
+      uint4 m4[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) m4[q] = *(const uint4*)&sm.mask[mw][row0 + 8 * q];
 #pragma unroll
       for (int e = 0; e < 16; e++) {
-        const int lrow = wr + rb * 32 + 8 * (e >> 2) + 4 * lh + (e & 3);
-        const int r = ti * kTile + lrow;
+        const int r = ti * kTile + row0 + 8 * (e >> 2) + (e & 3);
         const int c = tj * kTile + wc + cb * 32 + lr;
-        if (r < N && c < N && ((sm.mask[lrow][(wc + cb * 32) >> 5] >> lr) & 1u))
-          out[(long)r * N + c] = acc[rb][cb][e];
+        const uint4 m = m4[e >> 2];
+        const unsigned int word = (e & 3) == 0 ? m.x : (e & 3) == 1 ? m.y : (e & 3) == 2 ? m.z : m.w;
+        if (r < N && c < N && ((word >> lr) & 1u)) out[(long)r * N + c] = acc[rb][cb][e];
       }
+    }
 }
 
 // c = w conj(V_r) of every visibility of slice blockIdx.y in polar form, (|c|,
@@ -761,6 +784,12 @@ extern "C" gvm_status gvm_set_fft_mode(gvm_ctx* ctx, int mode) {
   GVM_REQUIRE(!ctx->capturing, "gvm_set_fft_mode: capturing");
   if (ctx->fft_mode != mode) ctx->epoch++;
   ctx->fft_mode = mode;
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_adjoint_computed_pixels(const gvm_ctx* ctx, long* pixels) {
+  GVM_REQUIRE(ctx && pixels, "gvm_adjoint_computed_pixels: null argument");
+  *pixels = ctx->adjoint_mode == GVM_ADJOINT_DIRECT ? ctx->adj_px_direct : ctx->adj_px_gemm;
   return GVM_OK;
 }
 

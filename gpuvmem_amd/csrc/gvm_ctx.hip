@@ -348,6 +348,28 @@ static gvm_status create_impl(const gvm_geometry* g, const float* noise_host, gv
   for (long k = 0; k < n; k++)
     if (noise[k] < g->noise_cut) bits[k >> 5] |= 1u << (k & 31);
   if ((st = upload(&ctx->mask_bits, bits))) return st;
+  // the pixels the adjoint computes: the split-precision GEMM, the 128 x 64
+  // sub-tiles (of its 256 x 256 workgroup tiles) with a pixel on the mask;
+  // the direct form, its blocks of 256 consecutive pixels with one
+  {
+    const long wpr = N / 32;  // mask words per row
+    long gemm_px = 0;
+    for (int r0 = 0; r0 < N; r0 += 128)
+      for (int c0 = 0; c0 < N; c0 += 64) {
+        unsigned int any = 0u;
+        for (int r = r0; r < std::min(N, r0 + 128); r++)
+          for (int c = c0; c < std::min(N, c0 + 64); c += 32) any |= bits[(size_t)(r * wpr + c / 32)];
+        if (any) gemm_px += (long)(std::min(N, r0 + 128) - r0) * (std::min(N, c0 + 64) - c0);
+      }
+    long direct_px = 0;
+    for (long b = 0; b < n; b += 256) {
+      unsigned int any = 0u;
+      for (long k = b; k < std::min(n, b + 256); k += 32) any |= bits[(size_t)(k >> 5)];
+      if (any) direct_px += std::min(n, b + 256) - b;
+    }
+    ctx->adj_px_gemm = gemm_px;
+    ctx->adj_px_direct = direct_px;
+  }
   if ((st = dev_alloc(&ctx->grid, n))) return st;
   ctx->grid_slices = 1;
   if ((st = dev_alloc(&ctx->slice_tab, kMaxSlots))) return st;

@@ -83,6 +83,8 @@ struct gvm_ctx {
   // image-space tables
   float* noise = nullptr;       // float[M*N]
   unsigned int* mask_bits = nullptr;  // noise < noise_cut, one bit per pixel
+  long adj_px_gemm = 0;    // pixels the split-precision adjoint computes
+  long adj_px_direct = 0;  // pixels the direct-sum adjoint computes
   float* gcf = nullptr;         // gridding correction image float[M*N], or none
   float2* grid = nullptr;       // uv grid, complex float[N*M]
   float2* twiddle = nullptr;    // e^{+2 pi i k / N}, k < N

@@ -198,6 +198,15 @@ class Context:
         native.check(self.lib.gvm_get_split_vis(self.ctx, C.byref(v)), "gvm_get_split_vis")
         return v.value
 
+    def adjoint_computed_pixels(self) -> int:
+        """gvm_adjoint_computed_pixels: the pixels whose sums the adjoint of
+        the current form computes (the work of one launch is 4 x this x the
+        visibilities)."""
+        v = C.c_long()
+        native.check(self.lib.gvm_adjoint_computed_pixels(self.ctx, C.byref(v)),
+                     "gvm_adjoint_computed_pixels")
+        return v.value
+
     def autotune(self, candidates=None) -> dict:
         """gvm_autotune_adjoint: picks the adjoint's range length by timing
         (a chi2 must have run).  Returns the choice, the candidates' times and

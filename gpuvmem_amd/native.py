@@ -160,6 +160,7 @@ HIP_API = {
     "gvm_set_split_vis": [_P, _L],
     "gvm_set_fft_mode": [_P, _I],
     "gvm_get_split_vis": [_P, C.POINTER(_L)],
+    "gvm_adjoint_computed_pixels": [_P, C.POINTER(_L)],
     "gvm_autotune_adjoint": [_P, _P, _I, C.POINTER(_L), _DP, _IP],
     "gvm_weighting_apply_dev": [_P, _I, _F, C.POINTER(UVTaper), _I, _P, _P, _P, _P],
     "gvm_setup_beam_noise_dev": [_I, _P, _P, _P, _P, _D, _DP, _DP, _DP, _DP, _DP, _DP],

@@ -488,6 +488,13 @@ gvm_status gvm_set_split_vis(gvm_ctx* ctx, long split_vis);
  * single small image).  Both give identical grids. */
 gvm_status gvm_set_fft_mode(gvm_ctx* ctx, int mode);
 gvm_status gvm_get_split_vis(const gvm_ctx* ctx, long* split_vis);
+/* No counterpart in the reference (whose DChi2 computes the pixels on the
+ * mask, noise < noise_cut): the pixels whose sums the adjoint of the current
+ * form computes.  The split-precision GEMM computes the 128 x 64 sub-tiles of
+ * its workgroup tiles that hold a pixel on the mask and skips the others; the
+ * direct form, its blocks of 256 consecutive pixels that hold one.  The work
+ * of one adjoint launch is 4 x pixels x visibilities flop. */
+gvm_status gvm_adjoint_computed_pixels(const gvm_ctx* ctx, long* pixels);
 /* Autotuner: times the adjoint of the current residuals (a gvm_chi2 must
  * have run) for each candidate range length (NULL: 1536, 2048, 3072, 4096,
  * 6144, 8192, up to 3072 for images of 2048^2 and larger), with HIP events,

@@ -240,6 +240,13 @@ def test_full_size_bf16x3_whole_image_against_fp64(full):
     img[0] = np.maximum(p.sky / p.fg_scale * 0.5, p.minpix).astype(np.float32)
     ctx = Context(p, adjoint="bf16x3")
     assert ctx.split_vis() == 6144
+    # the GEMM computes the 128 x 64 sub-tiles that hold a pixel on the mask and
+    # skips the rest (a disc here: about 80 % of the image); the whole-image check
+    # below shows nothing on the mask is lost
+    on = p.noise < p.noise_cut
+    sub_on = on.reshape(p.N // 128, 128, p.M // 64, 64).any(axis=(1, 3))
+    assert ctx.adjoint_computed_pixels() == int(sub_on.sum()) * 128 * 64
+    assert on.sum() < ctx.adjoint_computed_pixels() < p.M * p.N
     d = ctx.image(img)
     ctx.chi2(d)
     res = ctx.zeros(2 * p.M * p.N)
