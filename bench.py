@@ -38,6 +38,9 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# the learned warm start of the 1024^2 workload (trained by bench.py --warm-start
+# --ws-workload 1 --ws-save-model)
+WARM_MODEL = ROOT / "gpuvmem_amd" / "data" / "warm_start_1024sq.npz"
 PEAK_TFLOPS = {"f16x3": 2500.0, "bf16x3": 2500.0, "f32": 157.3, "direct": 157.3}
 DTYPE = {
     "f16x3": "fp16 hi/lo x3 MFMA adjoint (fp32 accumulate, fp64 combine); fp32 forward",
@@ -251,18 +254,17 @@ def bench_gridding(p, steps: int, warmup: int, threads: int, adjoint: str) -> di
 
 
 def generate_pairs(seeds, iterations: int, which: int):
-    """Training pairs from the plain solver for `seeds`, in groups of 16 with
-    a progress line after each group."""
+    """Training pairs from the plain solver for `seeds`, with a progress line
+    after each pair."""
     import numpy as np
     from gpuvmem_amd import warmstart as W
 
     t0 = time.perf_counter()
     seeds = list(seeds)
     chunks = []
-    for c0 in range(0, len(seeds), 16):
-        chunks.append(W.make_pairs(seeds[c0:c0 + 16], iterations=iterations, which=which))
-        print(f"# pairs {c0 + len(chunks[-1].dirty)} ({time.perf_counter() - t0:.1f} s)",
-              file=sys.stderr, flush=True)
+    for c0 in range(len(seeds)):
+        chunks.append(W.make_pairs(seeds[c0:c0 + 1], iterations=iterations, which=which))
+        print(f"# pairs {c0 + 1} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     return W.Pairs(np.concatenate([c.dirty for c in chunks]),
                    np.concatenate([c.model for c in chunks]), seeds,
                    time.perf_counter() - t0)
@@ -270,13 +272,14 @@ def generate_pairs(seeds, iterations: int, which: int):
 
 def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, levels: int,
                      base: int, which: int = 0, batch: int = 8, load=None,
-                     eval_lbfgs: int = 0) -> dict:
+                     eval_lbfgs: int = 0, save_model=None) -> dict:
     """The learned warm start on observations of workload `which` (0: 256^2 /
     36,000 visibilities; 1: 1024^2 / 1,764,000; 4: 512^2 / 1,029,000): training pairs from the plain
     solver (CG-FRPRMN to its tolerance; generated here, or loaded from the
     files `load` that --ws-save runs wrote), U-Net training on the device
-    kernels, then the plain solver from the default image and from the
-    prediction on held-out observations."""
+    kernels (the trained network written to `save_model` when given), then
+    the plain solver from the default image and from the prediction on
+    held-out observations."""
     import numpy as np
     from gpuvmem_amd import warmstart as W
     from gpuvmem_amd.device import Context
@@ -301,6 +304,17 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
         f"# step {k} loss {loss:.3e} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
         flush=True))
     print(f"# trained in {t_train:.1f} s, loss {ws.history[-1][1]:.3e}", file=sys.stderr, flush=True)
+    if save_model:
+        ws.save(save_model, meta={
+            "workload": which, "image": [p.M, p.N], "visibilities": p.nvis,
+            "pairs": n_train, "pair_seeds": [min(pairs.seeds), max(pairs.seeds)],
+            "pair_solver": f"CG-FRPRMN on chi2+entropy, at most {iterations} iterations, "
+                           "bf16x3 adjoint",
+            "levels": levels, "base_channels": base, "parameters": ws.net.nparams,
+            "adam_steps": steps, "batch": batch, "learning_rate": 1e-3,
+            "final_loss": ws.history[-1][1], "train_seconds": t_train,
+            "pairs_seconds": t_pairs})
+        print(f"# network written to {save_model}", file=sys.stderr, flush=True)
     ev = dict(iterations=iterations)
     if eval_lbfgs:
         # the time-to-image objective and optimiser
@@ -423,7 +437,7 @@ def _slice_problem(p, k):
     return q
 
 
-def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
+def time_to_image(p, iterations: int, forms=("f32", "bf16x3"), warm_model=None) -> dict:
     """L-BFGS (memory 10) from the default initial image with each adjoint
     form, up to `iterations` iterations (the reference's optimiser may stop
     earlier when its line search makes no progress).  The target is the
@@ -432,12 +446,20 @@ def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
     context does one untimed evaluation first (workspace allocation).  Every
     form runs with the same settings: evaluation graphs replayed and the
     adjoint range length autotuned, so the ratio measures the adjoint's
-    precision form alone."""
+    precision form alone.
+
+    With `warm_model` (a network written by WarmStart.save), the last form
+    runs once more from the learned warm start: the dirty image, the
+    network's prediction and the fallback check (the default image when the
+    prediction's objective is not lower) are timed with the run.  Its target
+    is the value all runs reached."""
     from gpuvmem_amd.device import Context, Objective, fw_config
 
     terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
     out, runs, images = {}, {}, {}
-    for mode in forms:
+    legs = [(m, False) for m in forms] + ([(forms[-1], True)] if warm_model else [])
+    for mode, warm_leg in legs:
+        name = mode + "_warm" if warm_leg else mode
         extras = mode != "direct"
         ctx = Context(p, adjoint=mode)
         obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
@@ -447,16 +469,33 @@ def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
         obj.value(warm)
         tune = ctx.autotune() if extras else None
         obj.gradient(warm, xi, 0)
-        ctx.sync()
-        d = ctx.image(p.initial_image())
+        rec = {}
+        if warm_leg:
+            from gpuvmem_amd import warmstart as W
+
+            ws = W.WarmStart.load(ctx, warm_model, batch=1)
+            W.start_image(ws, ctx, obj, p)[0].free()  # untimed first run of the network
+            ctx.sync()
+            t0 = time.perf_counter()
+            d, used, f_start = W.start_image(ws, ctx, obj, p)
+            ctx.sync()
+            t_start = time.perf_counter() - t0
+            rec = {"prediction_used": used, "start_objective": f_start,
+                   "start_seconds": t_start, "network": ws.meta | {"file": Path(warm_model).name}}
+            ws.close()
+        else:
+            ctx.sync()
+            d = ctx.image(p.initial_image())
+            f_start, t_start = obj.value(d), 0.0
         res, traj, secs = obj.optimize_timed(d, "CG-LBFGS", iterations, lbfgs_k=10)
         obj.value(d)
         chi2 = obj.term_values()[0]
-        images[mode] = d.download(np.float32, (2, p.N, p.M))
-        runs[mode] = (traj, secs)
-        out[mode] = {
+        images[name] = d.download(np.float32, (2, p.N, p.M))
+        # the starting point first, then one entry per iteration
+        runs[name] = ([f_start] + list(traj), [t_start] + [t_start + s for s in secs])
+        out[name] = {
             "iterations_run": res.iterations,
-            "seconds_run": res.seconds,
+            "seconds_run": t_start + res.seconds,
             "seconds_per_iteration": res.seconds / max(1, res.iterations),
             "final_objective": res.final_value,
             "final_chi2": chi2,
@@ -466,45 +505,59 @@ def time_to_image(p, iterations: int, forms=("f32", "bf16x3")) -> dict:
             "graph_replays": obj.graph_launches(),
             "split_vis": ctx.split_vis() if mode != "direct" else None,
             "autotune": tune,
+            **rec,
         }
         obj.close()
         ctx.close()
+    names = list(runs)
+    # the split ratio's target: the value the default-start forms reached
     target = max(runs[m][0][-1] for m in forms)
-    for mode in forms:
-        traj, secs = runs[mode]
-        hit = next(k for k, v in enumerate(traj) if v <= target)
-        out[mode]["seconds_to_target"] = secs[hit]
-        out[mode]["iterations_to_target"] = hit + 1
-        print(f"# time-to-image {mode}: {out[mode]}", file=sys.stderr, flush=True)
+    target_all = max(runs[m][0][-1] for m in names)
+    for name in names:
+        traj, secs = runs[name]
+        for key, tg in (("", target), ("_all", target_all)):
+            if key == "" and name not in forms:
+                continue
+            hit = next(k for k, v in enumerate(traj) if v <= tg)
+            out[name]["seconds_to_target" + key] = secs[hit]
+            out[name]["iterations_to_target" + key] = hit
+        print(f"# time-to-image {name}: {out[name]}", file=sys.stderr, flush=True)
     ref = images[forms[0]]
-    for mode in forms[1:]:
-        out[mode]["image_rel_l2_vs_" + forms[0]] = float(
-            np.linalg.norm(images[mode] - ref) / np.linalg.norm(ref))
-    return {"target_objective": target, "forms": out}
+    for name in names[1:]:
+        out[name]["image_rel_l2_vs_" + forms[0]] = float(
+            np.linalg.norm(images[name] - ref) / np.linalg.norm(ref))
+    return {"target_objective": target, "target_objective_all": target_all, "forms": out}
 
 
-def tti_observations(p, workload: int, iterations: int, n_obs: int) -> dict:
+def tti_observations(p, workload: int, iterations: int, n_obs: int, warm_model=None) -> dict:
     """time_to_image on n_obs observations (new sky and noise seeds, same
     array): the line search is chaotic, so where each form stops, and the
     ratio at the common objective, vary from one observation to the next.
     Returns the median ratio of plain FP32 seconds to split-precision
-    seconds with the per-observation records."""
+    seconds with the per-observation records, and with `warm_model` the
+    median ratio of plain FP32 seconds to split-precision seconds from the
+    learned warm start, both to the value all runs reached."""
     obs = []
     for k in range(max(1, n_obs)):
         pk = p if k == 0 else make_problem_of(workload, seed_sky=p.sim.seed_sky + 1000 * k,
                                               seed_noise=p.sim.seed_noise + 1000 * k)
-        r = time_to_image(pk, iterations)
+        r = time_to_image(pk, iterations, warm_model=warm_model)
         f32, split = r["forms"]["f32"], r["forms"]["bf16x3"]
         r["ratio"] = (f32["seconds_to_target"] / split["seconds_to_target"]
                       if split["seconds_to_target"] else None)
         r["per_iteration_ratio"] = (f32["seconds_per_iteration"] /
                                     split["seconds_per_iteration"])
         r["seed_sky"] = int(pk.sim.seed_sky)
-        print(f"# observation {k}: ratio {r['ratio']:.3f}, per iteration "
-              f"{r['per_iteration_ratio']:.3f}", file=sys.stderr, flush=True)
+        msg = f"# observation {k}: ratio {r['ratio']:.3f}, per iteration {r['per_iteration_ratio']:.3f}"
+        if warm_model:
+            w = r["forms"]["bf16x3_warm"]
+            r["ratio_warm"] = f32["seconds_to_target_all"] / w["seconds_to_target_all"]
+            r["ratio_split_all"] = f32["seconds_to_target_all"] / split["seconds_to_target_all"]
+            msg += f", warm start {r['ratio_warm']:.3f}"
+        print(msg, file=sys.stderr, flush=True)
         obs.append(r)
     ratios = [o["ratio"] for o in obs if o["ratio"]]
-    return {
+    out = {
         "value": float(np.median(ratios)) if ratios else None,
         "unit": "x (plain FP32 seconds / split-precision seconds, median over observations)",
         "config": {"workload": f"{p.M}x{p.N} image, {p.nvis} visibilities, chi2+entropy+TSV, "
@@ -514,6 +567,25 @@ def tti_observations(p, workload: int, iterations: int, n_obs: int) -> dict:
         "median_per_iteration_ratio": float(np.median([o["per_iteration_ratio"] for o in obs])),
         "observations": obs,
     }
+    if warm_model:
+        ws = [o["forms"]["bf16x3_warm"] for o in obs]
+        out["warm_start"] = {
+            "value": float(np.median([o["ratio_warm"] for o in obs])),
+            "unit": "x (plain FP32 seconds / split-precision + graphs + learned warm start "
+                    "seconds, to the objective all runs reached; median over observations)",
+            "median_ratio_split_same_target": float(np.median([o["ratio_split_all"]
+                                                               for o in obs])),
+            "median_iterations": {
+                "f32": float(np.median([o["forms"]["f32"]["iterations_to_target_all"]
+                                        for o in obs])),
+                "bf16x3": float(np.median([o["forms"]["bf16x3"]["iterations_to_target_all"]
+                                           for o in obs])),
+                "bf16x3_warm": float(np.median([w["iterations_to_target_all"] for w in ws]))},
+            "prediction_used": sum(bool(w["prediction_used"]) for w in ws),
+            "median_start_seconds": float(np.median([w["start_seconds"] for w in ws])),
+            "network": ws[0]["network"],
+        }
+    return out
 
 
 def make_problem_of(workload: int, **overrides):
@@ -582,6 +654,9 @@ def main() -> int:
                     help="time-to-image: observations (sky and noise seeds) to take the median over")
     ap.add_argument("--tti-iterations", type=int, default=100,
                     help="time-to-image leg of the default run: L-BFGS iteration cap per form")
+    ap.add_argument("--tti-warm-model", metavar="FILE", default=str(WARM_MODEL),
+                    help="time to image: also from the learned warm start of this network "
+                         "(written by --ws-save-model; 'none' to skip)")
     ap.add_argument("--no-tti", action="store_true",
                     help="default run: leave out the time-to-image and plain-FP32 legs")
     ap.add_argument("--time-to-image", type=int, default=0, metavar="ITERS",
@@ -621,6 +696,8 @@ def main() -> int:
                     help="warm start: sky seeds [A, B) of the pairs that --ws-save generates")
     ap.add_argument("--ws-load", nargs="+", metavar="FILE",
                     help="warm start: train on the pairs of these files instead of generating")
+    ap.add_argument("--ws-save-model", metavar="FILE", default=None,
+                    help="warm start: write the trained network to FILE (.npz)")
     ap.add_argument("--unet-step", action="store_true",
                     help="time training steps (forward, MSE, backward, Adam) of the warm-start "
                          "network at its size: 6 levels, 32 base channels, 512^2, batch 8, BF16 "
@@ -660,7 +737,7 @@ def main() -> int:
                              levels=args.ws_levels or (4 if args.ws_workload == 0 else 6),
                              base=args.ws_base or (32 if args.ws_workload == 4 else 16),
                              which=args.ws_workload, batch=args.ws_batch, load=args.ws_load,
-                             eval_lbfgs=args.ws_eval_lbfgs)
+                             eval_lbfgs=args.ws_eval_lbfgs, save_model=args.ws_save_model)
         print(json.dumps(r), flush=True)
         return 0
     t0 = time.perf_counter()
@@ -681,7 +758,8 @@ def main() -> int:
                           **r}), flush=True)
         return 0
     if args.time_to_image > 0:
-        r = tti_observations(p, args.workload, args.time_to_image, args.tti_observations)
+        r = tti_observations(p, args.workload, args.time_to_image, args.tti_observations,
+                             warm_model=tti_warm_model(args, p))
         print(json.dumps({
             "metric": "time-to-image vs plain FP32 at equal objective (1024^2, 1.76M vis)",
             "higher_is_better": True, "n_gpus": 1, "data": "synthetic", **r}), flush=True)
@@ -787,12 +865,27 @@ def main() -> int:
         # the plain FP32 form's evals/s on the same step, then time to image
         line["plain_fp32"] = plain_fp32_step(p, args.steps, args.warmup, args.graphs)
         print(f"# plain FP32 step: {line['plain_fp32']}", file=sys.stderr, flush=True)
-        tti = tti_observations(p, args.workload, args.tti_iterations, args.tti_observations)
+        tti = tti_observations(p, args.workload, args.tti_iterations, args.tti_observations,
+                               warm_model=tti_warm_model(args, p))
         line["time_to_image"] = compact_tti(tti)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(p, host_cpus(), args.cpu_seconds)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def tti_warm_model(args, p):
+    """The network for the warm-started time-to-image leg: --tti-warm-model
+    when the file exists and was trained at this image size, else None."""
+    f = args.tti_warm_model
+    if not f or f.lower() == "none" or not Path(f).is_file():
+        return None
+    with np.load(f, allow_pickle=False) as z:
+        if int(z["npix"]) != p.M or p.M != p.N:
+            print(f"# {f}: trained at {int(z['npix'])}^2, not {p.M}^2; no warm-start leg",
+                  file=sys.stderr, flush=True)
+            return None
+    return f
 
 
 def value_latency(obj, d_img, ctx, n: int = 200) -> dict:
@@ -850,15 +943,19 @@ def compact_tti(r: dict) -> dict:
         rec = {"seed_sky": o["seed_sky"], "ratio": o["ratio"],
                "per_iteration_ratio": o["per_iteration_ratio"],
                "target_objective": o["target_objective"]}
+        if "ratio_warm" in o:
+            rec["ratio_warm"] = o["ratio_warm"]
+            rec["target_objective_all"] = o["target_objective_all"]
         for m, f in o["forms"].items():
             rec[m] = {k: f[k] for k in ("seconds_to_target", "iterations_to_target",
+                                        "seconds_to_target_all", "iterations_to_target_all",
                                         "iterations_run", "seconds_run", "final_objective",
-                                        "final_chi2", "split_vis") if k in f}
-            if "image_rel_l2_vs_f32" in f:
-                rec[m]["image_rel_l2_vs_f32"] = f["image_rel_l2_vs_f32"]
+                                        "final_chi2", "split_vis", "prediction_used",
+                                        "start_objective", "start_seconds",
+                                        "image_rel_l2_vs_f32") if k in f}
         obs.append(rec)
-    return {k: r[k] for k in ("value", "unit", "config", "median_per_iteration_ratio")} | {
-        "observations": obs}
+    keep = ("value", "unit", "config", "median_per_iteration_ratio", "warm_start")
+    return {k: r[k] for k in keep if k in r} | {"observations": obs}
 
 
 if __name__ == "__main__":

@@ -18,6 +18,7 @@ calibration constant (learned from the training set).
 from __future__ import annotations
 
 import ctypes as C
+import json
 import time
 from dataclasses import dataclass, field
 from typing import Optional
@@ -261,6 +262,22 @@ class UNet:
             out[name] = (wflat[wo:bo].reshape(co, ci, k, k), flat[bo:bo + co])
         return out
 
+    def bias_mask(self) -> np.ndarray:
+        """True at the bias entries of the flat parameter vector."""
+        m = np.zeros(self.nparams, bool)
+        for wo, bo, co, _, _ in self.offsets.values():
+            m[bo:bo + co] = True
+        return m
+
+    def set_params(self, flat: np.ndarray) -> None:
+        """Replaces the FP32 master parameters (flat, storage order) and
+        refreshes the BF16 copy the convolutions read."""
+        flat = np.ascontiguousarray(flat, np.float32)
+        if flat.shape != (self.nparams,):
+            raise ValueError(f"set_params: {flat.shape} for {self.nparams} parameters")
+        self.params.upload(flat)
+        self._refresh_bf16()
+
     def grads_dict(self) -> dict:
         self.ctx.sync()
         flat = self.grads.download(np.float32).astype(np.float64)
@@ -369,6 +386,7 @@ class WarmStart:
         self.ctx = ctx
         self.calib = 1.0
         self.history: list = []
+        self.meta: dict = {}
 
     def _scale(self, dirty):
         return np.maximum(np.abs(dirty).reshape(len(dirty), -1).max(axis=1), 1e-30)
@@ -416,8 +434,71 @@ class WarmStart:
             out.extend(self.net.output(len(xk))[:, 0])
         return np.stack(out) * (s * self.calib)[:, None, None]
 
+    def save(self, path, meta: Optional[dict] = None) -> None:
+        """Writes the network as its convolutions use it to an .npz file that
+        np.load reads without pickle: the BF16 weight bits and the FP32
+        biases (every parameter in FP32 for a network with FP32 weights), the
+        shape, the calibration constant and `meta` (a JSON object: how the
+        network was trained)."""
+        net = self.net
+        self.ctx.sync()
+        flat = net.params.download(np.float32)[: net.nparams]
+        bias = net.bias_mask()
+        arrays = dict(levels=np.int64(net.levels), base=np.int64(net.base),
+                      npix=np.int64(net.H), calib=np.float64(self.calib), bf16=np.bool_(net.bf16),
+                      meta=np.str_(json.dumps(meta or {})))
+        if net.bf16:
+            bits = net.params_bf16.download(np.uint16)[: net.nparams]
+            arrays["weights_bf16"] = bits[~bias]
+            arrays["biases"] = flat[bias]
+        else:
+            arrays["params"] = flat
+        np.savez_compressed(path, **arrays)
+
+    @classmethod
+    def load(cls, ctx: Context, path, batch: int = 1, npix: Optional[int] = None) -> "WarmStart":
+        """A network written by save(), for batches of `batch` images of
+        npix x npix (default: the size it was trained at).  Its predictions
+        equal those of the network that was saved, bit for bit."""
+        with np.load(path, allow_pickle=False) as z:
+            levels, base = int(z["levels"]), int(z["base"])
+            ws = cls(ctx, levels=levels, base=base, batch=batch,
+                     npix=int(z["npix"]) if npix is None else int(npix), bf16=bool(z["bf16"]))
+            net = ws.net
+            if net.bf16:
+                bias = net.bias_mask()
+                flat = np.empty(net.nparams, np.float32)
+                flat[~bias] = (z["weights_bf16"].astype(np.uint32) << 16).view(np.float32)
+                flat[bias] = z["biases"]
+            else:
+                flat = z["params"]
+            net.set_params(flat)
+            ws.calib = float(z["calib"])
+            ws.meta = json.loads(str(z["meta"]))
+        return ws
+
     def close(self) -> None:
         self.net.close()
+
+
+def start_image(ws: WarmStart, ctx: Context, obj: Objective, p):
+    """Where the MEM optimisation starts: the network's prediction from the
+    dirty image (clipped to minpix, alpha 0) when the objective `obj` is
+    lower there than at the default image, else the default image, the
+    fallback.  Returns (device image, True when the prediction is used,
+    objective value at the start)."""
+    dirty = dirty_image(ctx, p)
+    img = p.initial_image()
+    img[0] = np.maximum(ws.predict(dirty[None])[0], p.minpix).astype(np.float32)
+    d = ctx.image(img)
+    f_pred = obj.value(d)
+    d0 = ctx.image(p.initial_image())
+    f_default = obj.value(d0)
+    if f_pred < f_default:
+        d0.free()
+        return d, True, f_pred
+    d.free()
+    return d0, False, f_default
 
 
 def time_to(traj, secs, target: float):
