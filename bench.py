@@ -15,7 +15,10 @@ reference's DChi2 computes the pixels on the mask) per launch, over its
 average duration (HIP events on the context's stream).  The cpu_baseline leg
 times the FP64 oracle on the host cores on a bounded sample (full-size
 forward chi2 + the separable FP64 gradient over a slice of the
-visibilities, extrapolated linearly in the visibility count).
+visibilities, extrapolated linearly in the visibility count).  The
+time_to_image object times L-BFGS to a common objective with the plain FP32
+and the split-precision adjoint, and the split form once more from the
+learned warm start of the committed network (gpuvmem_amd/data/).
 
 Other modes, each printing its own JSON line: --workload 0/2/3 (other
 workloads), --time-to-image ITERS (split precision against plain FP32 at
