@@ -3,9 +3,9 @@
  * objective, gradient and optimiser trajectory.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
- * (or the timed CPU baseline).  The product path (gpuvmem_amd/) never links,
- * loads or calls it.
+ * cpu_baseline leg of bench.py may load this library, and only to check
+ * results against (or as the timed CPU baseline).  The product path
+ * (gpuvmem_amd/) never links, loads or calls it.
  *
  * Every function cites the reference file:line it restates.  Arithmetic is
  * double precision throughout; inputs that are float in the reference (images,

@@ -1,8 +1,8 @@
 """ctypes wrapper of the FP64 CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
-the cpu_baseline leg of bench.py, always as the checker (or as the timed CPU
-baseline), never by the product path.  Every function restates the reference
+the cpu_baseline leg of bench.py, always to check results against (or as the
+timed CPU baseline), never by the product path.  Every function restates the reference
 (file:line citations are in gvm_oracle.c).
 """
 from __future__ import annotations
