@@ -318,12 +318,13 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
             "final_loss": ws.history[-1][1], "train_seconds": t_train,
             "pairs_seconds": t_pairs})
         print(f"# network written to {save_model}", file=sys.stderr, flush=True)
-    ev = dict(iterations=iterations)
-    if eval_lbfgs:
-        # the time-to-image objective and optimiser
-        ev = dict(iterations=eval_lbfgs, optimizer="CG-LBFGS",
-                  terms=[("Chi2", -1, 0.0), ("Entropy", 0, 0.001),
-                         ("TotalSquaredVariation", 1, 0.0)], lambdas=(0.01, 1e-3))
+    if n_test <= 0:
+        out = {"trained": True, "pairs": n_train, "pairs_seconds": t_pairs,
+               "train_seconds": t_train, "train_loss": ws.history, "model": save_model}
+        ws.close()
+        ctx.close()
+        return out
+    ev = eval_settings(iterations, eval_lbfgs)
     r = W.evaluate(ws, range(5000, 5000 + n_test), plain=True, which=which, **ev,
                    progress=lambda row: print(
                        f"# held-out {row['seed']}: iterations {row['iters_default']} / "
@@ -352,6 +353,43 @@ def bench_warm_start(n_train: int, n_test: int, steps: int, iterations: int, lev
     ws.close()
     ctx.close()
     return out
+
+
+def eval_settings(iterations: int, eval_lbfgs: int) -> dict:
+    """The held-out solves: the pair solver (CG-FRPRMN on chi2 + entropy, at
+    most `iterations`), or with eval_lbfgs the time-to-image objective and
+    optimiser (L-BFGS m = 10 on chi2 + entropy + TSV, at most eval_lbfgs)."""
+    if eval_lbfgs:
+        return dict(iterations=eval_lbfgs, optimizer="CG-LBFGS",
+                    terms=[("Chi2", -1, 0.0), ("Entropy", 0, 0.001),
+                           ("TotalSquaredVariation", 1, 0.0)], lambdas=(0.01, 1e-3))
+    return dict(iterations=iterations)
+
+
+def bench_warm_eval(model: str, seeds, which: int, iterations: int, eval_lbfgs: int = 0) -> dict:
+    """A network written by --ws-save-model on the held-out observations
+    `seeds` of workload `which`: the plain solver from the default image and
+    from the prediction, iterations and seconds to the objective both reach
+    (dirty image and inference counted for the warm start), one record per
+    observation."""
+    from gpuvmem_amd import warmstart as W
+    from gpuvmem_amd.device import Context
+
+    seeds = list(seeds)
+    p = W.observation(seeds[0], which)
+    ctx = Context(p, add_data=False)
+    ws = W.WarmStart.load(ctx, model, batch=1, npix=p.M)
+    r = W.evaluate(ws, seeds, which=which, **eval_settings(iterations, eval_lbfgs),
+                   progress=lambda row: print(
+                       f"# held-out {row['seed']}: iterations {row['iters_default']} / "
+                       f"{row['iters_warm']}, seconds {row['seconds_default']:.2f} / "
+                       f"{row['seconds_warm']:.2f}", file=sys.stderr, flush=True))
+    r["network"] = ws.meta | {"file": Path(model).name}
+    r["rows"] = [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in row.items()}
+                 for row in r["rows"]]
+    ws.close()
+    ctx.close()
+    return r
 
 
 def bench_unet_step(steps: int, warmup: int, levels: int = 6, base: int = 32, batch: int = 8,
@@ -700,7 +738,11 @@ def main() -> int:
     ap.add_argument("--ws-load", nargs="+", metavar="FILE",
                     help="warm start: train on the pairs of these files instead of generating")
     ap.add_argument("--ws-save-model", metavar="FILE", default=None,
-                    help="warm start: write the trained network to FILE (.npz)")
+                    help="warm start: write the trained network to FILE (.npz); "
+                         "with --ws-test 0, train and write only")
+    ap.add_argument("--ws-eval-model", metavar="FILE", default=None,
+                    help="warm start: only evaluate the network in FILE on the held-out "
+                         "observations --ws-seeds A:B")
     ap.add_argument("--unet-step", action="store_true",
                     help="time training steps (forward, MSE, backward, Adam) of the warm-start "
                          "network at its size: 6 levels, 32 base channels, 512^2, batch 8, BF16 "
@@ -726,6 +768,12 @@ def main() -> int:
             from gpuvmem_amd import warmstart
 
             warmstart.GRAPHS = False
+        if args.ws_eval_model:
+            a, b = (int(x) for x in (args.ws_seeds or "5000:%d" % (5000 + args.ws_test)).split(":"))
+            r = bench_warm_eval(args.ws_eval_model, range(a, b), args.ws_workload,
+                                args.ws_iterations, args.ws_eval_lbfgs)
+            print(json.dumps(r), flush=True)
+            return 0
         if args.ws_save:
             a, b = (int(x) for x in (args.ws_seeds or "1000:%d" % (1000 + args.ws_pairs)).split(":"))
             pr = generate_pairs(range(a, b), args.ws_iterations, args.ws_workload)
