@@ -680,6 +680,59 @@ def precision_study(p, pixels: int = 64) -> dict:
     return {"pixels": pixels, "forms": out}
 
 
+def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/1536")) -> dict:
+    """The image after `iterations` CG-FRPRMN iterations (chi2 + entropy +
+    TSV from the default image) with the plain FP32 adjoint, its applied step
+    lengths recorded, then the same path in each other form with those steps
+    replayed: what separates the final images is then the adjoint's
+    arithmetic, not the line search's branch decisions.  Reports each form's
+    final image (inside the mask) and objective values against the FP32
+    run's.  A form "mode/L" runs with visibility ranges of L: "f32/1536" is
+    the FP32 form with only its accumulation order changed, which shows how
+    far the path carries a rounding-level difference in the gradient."""
+    from gpuvmem_amd.device import Context, Objective, fw_config
+
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
+    mask = p.noise < p.noise_cut
+    runs = {}
+    for mode in ("f32",) + tuple(forms):
+        form, _, rng = mode.partition("/")
+        ctx = Context(p, adjoint=form)
+        if rng:
+            ctx.set_split_vis(int(rng))
+        obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
+        if mode != "f32":
+            obj.set_line_search_replay(runs["f32"]["steps"])
+        d = ctx.image(p.initial_image())
+        n = iterations if mode == "f32" else len(runs["f32"]["steps"])
+        t0 = time.perf_counter()
+        res, traj = obj.optimize(d, "CG-FRPRMN", n)
+        sec = time.perf_counter() - t0
+        runs[mode] = {"res": res, "traj": np.array(traj), "steps": obj.line_search_steps(),
+                      "img": d.download(np.float32, (2, p.N, p.M))[0][mask].astype(np.float64),
+                      "seconds": sec}
+        print(f"# replay study {mode}: {res.iterations} iterations, f {res.final_value:.10e}, "
+              f"{sec:.1f} s", file=sys.stderr, flush=True)
+        obj.close()
+        ctx.close()
+    ref = runs["f32"]
+    out = {"reference": {"form": "f32", "iterations": ref["res"].iterations,
+                         "final_objective": ref["res"].final_value, "seconds": ref["seconds"]}}
+    for mode in forms:
+        r = runs[mode]
+        k = min(len(r["traj"]), len(ref["traj"]))
+        out[mode] = {
+            "iterations": r["res"].iterations,
+            "final_image_rel_l2_vs_f32": float(np.linalg.norm(r["img"] - ref["img"]) /
+                                               np.linalg.norm(ref["img"])),
+            "final_objective_rel_vs_f32": abs(r["res"].final_value - ref["res"].final_value) /
+                                          abs(ref["res"].final_value),
+            "max_path_value_rel_vs_f32": float(np.max(np.abs(r["traj"][:k] - ref["traj"][:k]) /
+                                                      np.abs(ref["traj"][:k]))),
+            "seconds_replayed": r["seconds"]}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -703,6 +756,9 @@ def main() -> int:
     ap.add_argument("--time-to-image", type=int, default=0, metavar="ITERS",
                     help="instead of evals/s: L-BFGS time-to-image of the split-precision "
                          "adjoint against the plain FP32 one at equal objective value")
+    ap.add_argument("--study-iterations", type=int, default=0, metavar="ITERS",
+                    help="precision study: also the image after ITERS CG-FRPRMN iterations, "
+                         "FP32 path replayed in the split forms")
     ap.add_argument("--precision-study", action="store_true",
                     help="instead of evals/s: each adjoint form against FP64 (use with "
                          "--workload 2: 2048^2, 4.35M vis)")
@@ -798,6 +854,8 @@ def main() -> int:
           f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     if args.precision_study:
         r = precision_study(p)
+        if args.study_iterations > 0:
+            r["replayed_path"] = replay_image_study(p, args.study_iterations)
         print(json.dumps({"metric": f"adjoint precision against FP64 ({p.M}^2, {p.nvis_total} vis)",
                           "n_gpus": 1, "data": "synthetic", **r}), flush=True)
         return 0

@@ -21,6 +21,7 @@ struct gvm_fw_objective {
   imageMap* mapping = nullptr;
   bool lbfgs_host_scalars = false;
   std::vector<double> replay;  // line-search step lengths to replay (empty: search)
+  std::vector<double> steps;   // the step lengths the last optimisation applied
   ~gvm_fw_objective() { delete[] mapping; }
 };
 
@@ -153,7 +154,9 @@ extern "C" gvm_status gvm_fw_optimize_timed(gvm_fw_objective* obj, const char* o
     struct ReplayReset {
       ~ReplayReset() { setLineSearchReplay(nullptr, 0); }
     } replay_reset;
+    obj->steps.clear();
     opt->optimize();
+    obj->steps = lineSearchSteps();
     gvm::check(gvm_sync(obj->ctx), "gvm_sync");
     const auto t1 = std::chrono::steady_clock::now();
     res->iterations = opt->getCurrentIteration();
@@ -183,6 +186,17 @@ extern "C" gvm_status gvm_fw_set_line_search_replay(gvm_fw_objective* obj, const
     return GVM_ERR_ARG;
   }
   obj->replay.assign(xmins, xmins + n);
+  return GVM_OK;
+}
+
+extern "C" gvm_status gvm_fw_line_search_steps(const gvm_fw_objective* obj, double* xmins,
+                                                int cap, int* n) {
+  if (!obj || !n || cap < 0 || (cap > 0 && !xmins)) {
+    g_fw_error = "gvm_fw_line_search_steps: bad argument";
+    return GVM_ERR_ARG;
+  }
+  *n = (int)obj->steps.size();
+  for (int k = 0; k < *n && k < cap; k++) xmins[k] = obj->steps[k];
   return GVM_OK;
 }
 

@@ -264,12 +264,16 @@ double f1dim(double x) {
 namespace {
 std::vector<double> g_replay;
 size_t g_replay_pos = 0;
+std::vector<double> g_steps;  // the step lengths applied since the last reset
 }  // namespace
 
 void setLineSearchReplay(const double* xmins, int n) {
   g_replay.assign(xmins, xmins + std::max(0, n));
   g_replay_pos = 0;
+  g_steps.clear();
 }
+
+const std::vector<double>& lineSearchSteps() { return g_steps; }
 
 void linmin(float* p, float* xi, double* fret, f1dim_fn func) {
   const long tot = img_n() * globals().image_count;
@@ -286,6 +290,7 @@ void linmin(float* p, float* xi, double* fret, f1dim_fn func) {
     mnbrak(&ax, &xx, &bx, &fa, &fx, &fb, f1dim);
     *fret = brent(ax, xx, bx, 1.0e-7, &xmin, f1dim);
   }
+  g_steps.push_back(xmin);
   if (globals().verbose) std::printf("Alpha for linear minimization = %f\n\n", xmin);
   for (int i = 0; i < I->getImageCount(); i++)
     applyNewP(I->getFunctionMapping()[i], p, xi, xmin, i);

@@ -344,6 +344,16 @@ class Objective:
         native.check_fw(self.fw.gvm_fw_set_line_search_replay(self.h, ptr, int(arr.size)),
                         "gvm_fw_set_line_search_replay")
 
+    def line_search_steps(self) -> list[float]:
+        """The step lengths the last optimize() applied, in order."""
+        n = C.c_int()
+        native.check_fw(self.fw.gvm_fw_line_search_steps(self.h, None, 0, C.byref(n)),
+                        "gvm_fw_line_search_steps")
+        out = (C.c_double * max(1, n.value))()
+        native.check_fw(self.fw.gvm_fw_line_search_steps(self.h, out, n.value, C.byref(n)),
+                        "gvm_fw_line_search_steps")
+        return [out[k] for k in range(n.value)]
+
     def graph_launches(self) -> int:
         n = C.c_long()
         native.check_fw(self.fw.gvm_fw_graph_launches(self.h, C.byref(n)), "gvm_fw_graph_launches")

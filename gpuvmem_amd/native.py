@@ -245,6 +245,7 @@ FW_API = {
     "gvm_fw_set_lbfgs_host_scalars": ([_P, _I], _I),
     "gvm_fw_set_fused_terms": ([_P, _I], _I),
     "gvm_fw_set_line_search_replay": ([_P, C.POINTER(C.c_double), _I], _I),
+    "gvm_fw_line_search_steps": ([_P, C.POINTER(C.c_double), _I, C.POINTER(_I)], _I),
     "gvm_fw_graph_launches": ([_P, C.POINTER(_L)], _I),
     "gvm_fw_last_error": ([], C.c_char_p),
     "gvm_fw_registered": ([C.c_char_p, C.c_char_p], _I),

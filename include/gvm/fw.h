@@ -99,6 +99,11 @@ gvm_status gvm_fw_set_fused_terms(gvm_fw_objective* obj, int on);
  * steps another implementation accepted (the FP64 oracle's) compares two
  * optimiser paths without the line search's branch decisions. */
 gvm_status gvm_fw_set_line_search_replay(gvm_fw_objective* obj, const double* xmins, int n);
+/* The step lengths the last gvm_fw_optimize applied, searched or replayed (no
+ * counterpart in the reference): *n receives their count, xmins[0..min(n,
+ * cap)) the values.  Feeding them to gvm_fw_set_line_search_replay of another
+ * objective (another adjoint form) runs the same path there. */
+gvm_status gvm_fw_line_search_steps(const gvm_fw_objective* obj, double* xmins, int cap, int* n);
 const char* gvm_fw_last_error(void);
 
 /* 1 if `name` is registered with the factory of `kind` ("Fi", "Optimizer",

@@ -4,6 +4,8 @@
 // brent.cu, f1dim.cu).  Scalars are FP64; vectors stay on the device.
 #pragma once
 
+#include <vector>
+
 #include "gvm/optimizer.h"
 
 class ConjugateGradient : public Optimizer {
@@ -53,3 +55,6 @@ void linmin(float* p, float* xi, double* fret, f1dim_fn func);
 // Not in the reference: linmin takes its step lengths from xmins[0..n) in
 // turn instead of searching (n = 0 restores the search).
 void setLineSearchReplay(const double* xmins, int n);
+// Not in the reference: the step lengths linmin has applied since the last
+// setLineSearchReplay call, searched or replayed, in order.
+const std::vector<double>& lineSearchSteps();

@@ -180,6 +180,37 @@ def test_lbfgs_final_image_with_replayed_steps_workload0():
     assert np.all(terr < VALUE_PATH_TOL)
 
 
+def test_recorded_steps_replay_the_same_path():
+    """gvm_fw_line_search_steps: the step lengths a run applied, replayed in a
+    new objective of the same form, give the same trajectory and final image
+    bit for bit; replayed in the FP32 form they give the same path to within
+    the split form's arithmetic (final image within 1e-4)."""
+    p = small_problem(npix=128, n_antennas=8, seed=33)
+    print(f"case: recorded steps, CG-FRPRMN 20 it, {p.M}^2, nvis={p.nvis}", flush=True)
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001)]
+    runs = {}
+    for mode, steps in (("bf16x3", None), ("bf16x3", "replay"), ("f32", "replay")):
+        ctx = Context(p, adjoint=mode)
+        obj = Objective(ctx, fw_config(terms, lambdas=(0.01,)))
+        if steps:
+            obj.set_line_search_replay(runs["search"][2])
+        d = ctx.image(p.initial_image())
+        res, traj = obj.optimize(d, "CG-FRPRMN", runs["search"][0].iterations if steps else 20)
+        rec = (res, traj, obj.line_search_steps(), d.download(np.float32, (2, p.N, p.M)))
+        runs["search" if not steps else mode + "_replay"] = rec
+        obj.close()
+        ctx.close()
+    res, traj, steps, img = runs["search"]
+    assert len(steps) == res.iterations > 0
+    r2, t2, s2, i2 = runs["bf16x3_replay"]
+    assert s2 == steps and t2 == traj and np.array_equal(i2, img)
+    r3, t3, s3, i3 = runs["f32_replay"]
+    assert s3 == steps and r3.iterations == res.iterations
+    err = np.linalg.norm(i3 - img) / np.linalg.norm(img)
+    print(f"{res.iterations} steps; f32 replay final image rel L2 {err:.2e}", flush=True)
+    assert err < 1e-4
+
+
 def test_trajectory_is_bitwise_deterministic():
     """Fixed-order reductions: two runs give identical trajectories and images."""
     p = small_problem(npix=128, n_antennas=8, seed=31)
