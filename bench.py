@@ -680,16 +680,18 @@ def precision_study(p, pixels: int = 64) -> dict:
     return {"pixels": pixels, "forms": out}
 
 
-def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/1536")) -> dict:
+def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/6144")) -> dict:
     """The image after `iterations` CG-FRPRMN iterations (chi2 + entropy +
     TSV from the default image) with the plain FP32 adjoint, its applied step
     lengths recorded, then the same path in each other form with those steps
     replayed: what separates the final images is then the adjoint's
     arithmetic, not the line search's branch decisions.  Reports each form's
     final image (inside the mask) and objective values against the FP32
-    run's.  A form "mode/L" runs with visibility ranges of L: "f32/1536" is
-    the FP32 form with only its accumulation order changed, which shows how
-    far the path carries a rounding-level difference in the gradient."""
+    run's.  A form "mode/L" runs with visibility ranges of L: "f32/6144" is
+    the FP32 form with only its accumulation order changed (at 2048^2 the
+    default ranges are 3072 visibilities, and the workspace limit keeps them
+    from getting shorter), which shows how far the path carries a
+    rounding-level difference in the gradient."""
     from gpuvmem_amd.device import Context, Objective, fw_config
 
     terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
