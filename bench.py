@@ -710,16 +710,20 @@ def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/6144")
         t0 = time.perf_counter()
         res, traj = obj.optimize(d, "CG-FRPRMN", n)
         sec = time.perf_counter() - t0
+        img0 = d.download(np.float32, (2, p.N, p.M))[0][mask]
         runs[mode] = {"res": res, "traj": np.array(traj), "steps": obj.line_search_steps(),
-                      "img": d.download(np.float32, (2, p.N, p.M))[0][mask].astype(np.float64),
-                      "seconds": sec}
+                      "img": img0.astype(np.float64), "seconds": sec,
+                      # pixels held at the positivity floor of the image update
+                      "floor": img0 <= np.float32(p.minpix)}
         print(f"# replay study {mode}: {res.iterations} iterations, f {res.final_value:.10e}, "
               f"{sec:.1f} s", file=sys.stderr, flush=True)
         obj.close()
         ctx.close()
     ref = runs["f32"]
     out = {"reference": {"form": "f32", "iterations": ref["res"].iterations,
-                         "final_objective": ref["res"].final_value, "seconds": ref["seconds"]}}
+                         "final_objective": ref["res"].final_value, "seconds": ref["seconds"],
+                         "pixels_in_mask": int(mask.sum()),
+                         "pixels_at_floor": int(ref["floor"].sum())}}
     for mode in forms:
         r = runs[mode]
         k = min(len(r["traj"]), len(ref["traj"]))
@@ -731,6 +735,11 @@ def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/6144")
                                           abs(ref["res"].final_value),
             "max_path_value_rel_vs_f32": float(np.max(np.abs(r["traj"][:k] - ref["traj"][:k]) /
                                                       np.abs(ref["traj"][:k]))),
+            "pixels_at_floor": int(r["floor"].sum()),
+            "pixels_floor_differs": int((r["floor"] != ref["floor"]).sum()),
+            "final_image_rel_l2_off_floor": float(
+                np.linalg.norm((r["img"] - ref["img"])[~(r["floor"] | ref["floor"])]) /
+                np.linalg.norm(ref["img"][~(r["floor"] | ref["floor"])])),
             "seconds_replayed": r["seconds"]}
     return out
 
@@ -761,6 +770,8 @@ def main() -> int:
     ap.add_argument("--study-iterations", type=int, default=0, metavar="ITERS",
                     help="precision study: also the image after ITERS CG-FRPRMN iterations, "
                          "FP32 path replayed in the split forms")
+    ap.add_argument("--study-forms", default="bf16x3,f16x3,f32/6144",
+                    help="precision study: the forms that replay the FP32 path (mode or mode/L)")
     ap.add_argument("--precision-study", action="store_true",
                     help="instead of evals/s: each adjoint form against FP64 (use with "
                          "--workload 2: 2048^2, 4.35M vis)")
@@ -857,7 +868,8 @@ def main() -> int:
     if args.precision_study:
         r = precision_study(p)
         if args.study_iterations > 0:
-            r["replayed_path"] = replay_image_study(p, args.study_iterations)
+            r["replayed_path"] = replay_image_study(p, args.study_iterations,
+                                                    tuple(args.study_forms.split(",")))
         print(json.dumps({"metric": f"adjoint precision against FP64 ({p.M}^2, {p.nvis_total} vis)",
                           "n_gpus": 1, "data": "synthetic", **r}), flush=True)
         return 0
