@@ -710,9 +710,10 @@ def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/6144")
         t0 = time.perf_counter()
         res, traj = obj.optimize(d, "CG-FRPRMN", n)
         sec = time.perf_counter() - t0
-        img0 = d.download(np.float32, (2, p.N, p.M))[0][mask]
+        full = d.download(np.float32, (2, p.N, p.M))
+        img0 = full[0][mask]
         runs[mode] = {"res": res, "traj": np.array(traj), "steps": obj.line_search_steps(),
-                      "img": img0.astype(np.float64), "seconds": sec,
+                      "img": img0.astype(np.float64), "seconds": sec, "full": full,
                       # pixels held at the positivity floor of the image update
                       "floor": img0 <= np.float32(p.minpix)}
         print(f"# replay study {mode}: {res.iterations} iterations, f {res.final_value:.10e}, "
@@ -720,6 +721,39 @@ def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/6144")
         obj.close()
         ctx.close()
     ref = runs["f32"]
+    # the chi2 gradient at the end of the FP32 path, every form against FP32 over the mask
+    # and against FP64 direct sums on sampled pixels (as at the start of the path)
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    final = ref["full"]
+    orc = oracle.Oracle(p, separable=True)
+    _, vr, _ = orc.chi2(final.astype(np.float64))
+    inside = np.argwhere(mask)
+    pix = inside[np.random.default_rng(7).choice(len(inside), 64, replace=False)]
+    gref = orc.dchi2_pixels(vr, pix)
+    grads = {}
+    for mode in ("f32",) + tuple(forms):
+        form, _, rng = mode.partition("/")
+        ctx = Context(p, adjoint=form)
+        if rng:
+            ctx.set_split_vis(int(rng))
+        d = ctx.image(final)
+        ctx.chi2(d)
+        res = ctx.zeros(2 * p.M * p.N)
+        ctx.dchi2(d, res)
+        ctx.sync()
+        grads[mode] = res.download(np.float32, (2, p.N, p.M))[0].astype(np.float64)
+        ctx.close()
+    grad_final = {}
+    for mode, g in grads.items():
+        gs = np.array([g[i, j] for i, j in pix])
+        grad_final[mode] = {"rel_l2_vs_fp64_pixels": float(np.linalg.norm(gs - gref) /
+                                                           np.linalg.norm(gref))}
+        if mode != "f32":
+            grad_final[mode]["rel_l2_vs_f32_mask"] = float(
+                np.linalg.norm((g - grads["f32"])[mask]) / np.linalg.norm(grads["f32"][mask]))
+    out_grad = {"gradient_at_final_image": grad_final}
     out = {"reference": {"form": "f32", "iterations": ref["res"].iterations,
                          "final_objective": ref["res"].final_value, "seconds": ref["seconds"],
                          "pixels_in_mask": int(mask.sum()),
@@ -741,6 +775,7 @@ def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/6144")
                 np.linalg.norm((r["img"] - ref["img"])[~(r["floor"] | ref["floor"])]) /
                 np.linalg.norm(ref["img"][~(r["floor"] | ref["floor"])])),
             "seconds_replayed": r["seconds"]}
+    out.update(out_grad)
     return out
 
 
