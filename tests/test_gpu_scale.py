@@ -263,6 +263,38 @@ def test_full_size_bf16x3_whole_image_against_fp64(full):
     assert np.all(g[1] == 0)
 
 
+def test_full_size_split_forms_follow_the_fp32_path(full):
+    """CG-FRPRMN on chi2 + entropy + TSV at 1024^2 / 1,764,000 visibilities
+    with the plain FP32 adjoint, run to its own tolerance, its applied step
+    lengths replayed in the split forms (gvm_fw_line_search_steps): the
+    final images agree to 1e-5 relative L2 on the mask (measured 2.4e-7 and
+    2.5e-7; the FP64 oracle cannot run this path in a test's time)."""
+    from gpuvmem_amd.device import Objective, fw_config
+
+    p = full
+    print(f"case: full size FP32 CG path replayed in bf16x3 / f16x3, {p.M}^2", flush=True)
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
+    on = p.noise < p.noise_cut
+    out = {}
+    for mode in ("f32", "bf16x3", "f16x3"):
+        ctx = Context(p, adjoint=mode)
+        obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
+        if mode != "f32":
+            obj.set_line_search_replay(out["f32"][1])
+        d = ctx.image(p.initial_image())
+        res, _ = obj.optimize(d, "CG-FRPRMN", 100 if mode == "f32" else len(out["f32"][1]))
+        out[mode] = (res, obj.line_search_steps(), d.download(np.float32, (2, p.N, p.M))[0][on])
+        obj.close()
+        ctx.close()
+    ref = out["f32"][2].astype(np.float64)
+    assert len(out["f32"][1]) == out["f32"][0].iterations > 10
+    for mode in ("bf16x3", "f16x3"):
+        err = np.linalg.norm(out[mode][2] - ref) / np.linalg.norm(ref)
+        print(f"{mode}: {out[mode][0].iterations} of {out['f32'][0].iterations} steps, "
+              f"final image rel L2 {err:.2e}", flush=True)
+        assert err < 1e-5
+
+
 @pytest.mark.parametrize("mode", ["bf16x3", "f32"])
 def test_tiny_weights_keep_the_adjoint_scale_finite(mode):
     """Weights scaled so that the largest |c| = w |V_r| is about 2^-116: the
