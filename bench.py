@@ -779,6 +779,66 @@ def replay_image_study(p, iterations: int, forms=("bf16x3", "f16x3", "f32/6144")
     return out
 
 
+def gradient_difference_study(p, iterations: int,
+                              forms=("f32", "f32/6144", "bf16x3", "f16x3")) -> dict:
+    """How each form's gradient error changes between two consecutive
+    images of an optimisation path, against the change of the gradient
+    itself: the FP32 form's CG path (chi2 + entropy + TSV) gives image B after
+    `iterations` steps and, its first steps replayed, image A one step
+    before.  With the direct form as the reference g (its error is an order
+    below the others'), each form's error d = g_form - g is taken at A and
+    B on the mask; Polak-Ribiere's gamma and L-BFGS's y see
+    |d(B) - d(A)| / |g(B) - g(A)|."""
+    from gpuvmem_amd.device import Context, Objective, fw_config
+
+    terms = [("Chi2", -1, 0.0), ("Entropy", 0, 0.001), ("TotalSquaredVariation", 1, 0.0)]
+    mask = p.noise < p.noise_cut
+    imgs, steps = {}, None
+    for name, n in (("B", iterations), ("A", iterations - 1)):
+        ctx = Context(p, adjoint="f32")
+        obj = Objective(ctx, fw_config(terms, lambdas=(0.01, 1e-3)))
+        if steps is not None:
+            obj.set_line_search_replay(steps[:n])
+        d = ctx.image(p.initial_image())
+        obj.optimize(d, "CG-FRPRMN", n)
+        if steps is None:
+            steps = obj.line_search_steps()
+        imgs[name] = d.download(np.float32, (2, p.N, p.M))
+        obj.close()
+        ctx.close()
+    grads = {}
+    for mode in ("direct",) + tuple(forms):
+        form, _, rng = mode.partition("/")
+        ctx = Context(p, adjoint=form)
+        if rng:
+            ctx.set_split_vis(int(rng))
+        for name, img in imgs.items():
+            d = ctx.image(img)
+            ctx.chi2(d)
+            res = ctx.zeros(2 * p.M * p.N)
+            ctx.dchi2(d, res)
+            ctx.sync()
+            grads[mode, name] = res.download(np.float32, (2, p.N, p.M))[0][mask].astype(np.float64)
+        ctx.close()
+        print(f"# difference study: {mode} done", file=sys.stderr, flush=True)
+    gA, gB = grads["direct", "A"], grads["direct", "B"]
+    dg = np.linalg.norm(gB - gA)
+    out = {"iterations": iterations, "steps": len(steps),
+           "gradient_change_rel": float(dg / np.linalg.norm(gB))}
+    for mode in forms:
+        eA, eB = grads[mode, "A"] - gA, grads[mode, "B"] - gB
+        out[mode] = {"error_rel_at_B": float(np.linalg.norm(eB) / np.linalg.norm(gB)),
+                     # a systematic offset: the error's mean over its rms, and its
+                     # correlation with the gradient itself
+                     "error_mean_over_rms_at_B": float(eB.mean() / np.sqrt(np.mean(eB * eB))),
+                     "error_correlation_with_gradient_at_B": float(
+                         np.dot(eB, gB) / (np.linalg.norm(eB) * np.linalg.norm(gB))),
+                     "error_change_over_gradient_change": float(np.linalg.norm(eB - eA) / dg),
+                     "error_correlation_A_B": float(np.dot(eA, eB) /
+                                                    (np.linalg.norm(eA) * np.linalg.norm(eB)))}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -805,6 +865,9 @@ def main() -> int:
     ap.add_argument("--study-iterations", type=int, default=0, metavar="ITERS",
                     help="precision study: also the image after ITERS CG-FRPRMN iterations, "
                          "FP32 path replayed in the split forms")
+    ap.add_argument("--diff-study", type=int, default=0, metavar="ITERS",
+                    help="precision study: each form's gradient error between the last two "
+                         "images of an ITERS-step FP32 CG path, against the gradient's change")
     ap.add_argument("--study-forms", default="bf16x3,f16x3,f32/6144",
                     help="precision study: the forms that replay the FP32 path (mode or mode/L)")
     ap.add_argument("--precision-study", action="store_true",
@@ -902,6 +965,8 @@ def main() -> int:
           f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     if args.precision_study:
         r = precision_study(p)
+        if args.diff_study > 0:
+            r["gradient_difference"] = gradient_difference_study(p, args.diff_study)
         if args.study_iterations > 0:
             r["replayed_path"] = replay_image_study(p, args.study_iterations,
                                                     tuple(args.study_forms.split(",")))

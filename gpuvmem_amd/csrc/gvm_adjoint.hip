@@ -227,6 +227,15 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
   // are staged.  Rows and columns past the image produce finite values that
   // only reach outputs past the image, which are not written.
   const float scale = ldexpf(1.0f, p_scale_exponent(cm));
+  // Odd ranges accumulate -G (P negated, which the split and every product
+  // follow exactly); the epilogue negates their partials back.  The 16-bit
+  // MFMAs round their accumulation downwards, so each range's partial carries
+  // a bias that does not change sign with the terms: on a 2048^2 image it
+  // was nearly uniform over the pixels (mean / rms 0.87 for bf16x3, 0.92 for
+  // f16x3, 0.0002 for FP32), the same in every gradient of an optimisation.
+  // Alternating the sign flips that bias from one range to the next, so it
+  // cancels in the sum.
+  const float pscale = (split & 1) ? -scale : scale;
 
   const int lr = lane & 31, lh = lane >> 5;
   const int wr = (wave & 1) * 128, wc = (wave >> 1) * 64;
@@ -249,7 +258,7 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
   };
   auto store_rec = [&](int buf) {
     sm.rec_s[buf][slot] = rs;
-    sm.rec_d[buf][slot] = make_float4(rin ? rp.x * scale : 0.0f, rp.y, rw, 0.f);
+    sm.rec_d[buf][slot] = make_float4(rin ? rp.x * pscale : 0.0f, rp.y, rw, 0.f);
   };
 
   // ---- generation: P row gidx and Q column gidx for half of the chunk ----
@@ -538,8 +547,10 @@ __global__ void adjoint_epilogue_kernel(const float* __restrict__ part,
     for (int sl = 0; sl < nsl; sl++) {
       // FP64 sum over the slice's splits in split order; eight loads are
       // issued before their adds so each thread keeps several HBM requests
-      // in flight.
+      // in flight.  The GEMM's odd splits hold -G (their sign is flipped
+      // back here); the direct form's are not flipped.
       const int pa = p0[sl], pe = p0[sl + 1];
+      const int alt = scaled ? 1 : 0;
       double g = 0.0;
       int s = pa;
       for (; s + 8 <= pe; s += 8) {
@@ -547,9 +558,12 @@ __global__ void adjoint_epilogue_kernel(const float* __restrict__ part,
 #pragma unroll
         for (int u = 0; u < 8; u++) v[u] = part[(long)(s + u) * n + k];
 #pragma unroll
-        for (int u = 0; u < 8; u++) g += (double)v[u];
+        for (int u = 0; u < 8; u++) g += (double)(((s + u) & alt) ? -v[u] : v[u]);
       }
-      for (; s < pe; s++) g += (double)part[(long)s * n + k];
+      for (; s < pe; s++) {
+        const float v = part[(long)s * n + k];
+        g += (double)((s & alt) ? -v : v);
+      }
       double inv_scale = 1.0;
       if (scaled) {
         const float cm = __uint_as_float(cmaxs[sl]);
