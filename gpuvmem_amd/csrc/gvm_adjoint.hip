@@ -228,13 +228,13 @@ __global__ __launch_bounds__(kGemmThreads, 1) void adjoint_gemm_kernel(
   // only reach outputs past the image, which are not written.
   const float scale = ldexpf(1.0f, p_scale_exponent(cm));
   // Odd ranges accumulate -G (P negated, which the split and every product
-  // follow exactly); the epilogue negates their partials back.  The 16-bit
-  // MFMAs round their accumulation downwards, so each range's partial carries
-  // a bias that does not change sign with the terms: on a 2048^2 image it
-  // was nearly uniform over the pixels (mean / rms 0.87 for bf16x3, 0.92 for
-  // f16x3, 0.0002 for FP32), the same in every gradient of an optimisation.
-  // Alternating the sign flips that bias from one range to the next, so it
-  // cancels in the sum.
+  // follow exactly); the epilogue negates their partials back.  With the
+  // 16-bit MFMAs each range's partial carries an error that does not change
+  // sign with the terms, as a downward rounding of their accumulation would:
+  // on a 2048^2 image it was nearly uniform over the pixels (mean / rms 0.87
+  // for bf16x3, 0.92 for f16x3, 0.0002 for FP32) and the same in every
+  // gradient of an optimisation.  Alternating the sign flips that error from
+  // one range to the next, so it cancels in the sum.
   const float pscale = (split & 1) ? -scale : scale;
 
   const int lr = lane & 31, lh = lane >> 5;
