@@ -635,7 +635,7 @@ def make_problem_of(workload: int, **overrides):
     return make_problem(workload, **overrides)
 
 
-def precision_study(p, pixels: int = 64) -> dict:
+def precision_study(p, pixels: int = 64, split_vis: int = 0) -> dict:
     """Each adjoint form against FP64: chi2 against the oracle's FP64
     forward pass, the gradient against FP64 direct sums on a pixel sample
     inside the mask, and the split forms against the FP32 and direct forms
@@ -654,6 +654,8 @@ def precision_study(p, pixels: int = 64) -> dict:
     out, full = {}, {}
     for mode in ("direct", "f32", "f16x3", "bf16x3"):
         ctx = Context(p, adjoint=mode)
+        if split_vis and mode != "direct":
+            ctx.set_split_vis(split_vis)
         d = ctx.image(img)
         v = ctx.chi2(d)
         res = ctx.zeros(2 * p.M * p.N)
@@ -865,6 +867,9 @@ def main() -> int:
     ap.add_argument("--study-iterations", type=int, default=0, metavar="ITERS",
                     help="precision study: also the image after ITERS CG-FRPRMN iterations, "
                          "FP32 path replayed in the split forms")
+    ap.add_argument("--split-vis", type=int, default=0, metavar="L",
+                    help="precision study: visibility range length of the GEMM forms "
+                         "(default: the context's)")
     ap.add_argument("--diff-study", type=int, default=0, metavar="ITERS",
                     help="precision study: each form's gradient error between the last two "
                          "images of an ITERS-step FP32 CG path, against the gradient's change")
@@ -964,7 +969,7 @@ def main() -> int:
           f"{p.nvis_total} visibilities "
           f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     if args.precision_study:
-        r = precision_study(p)
+        r = precision_study(p, split_vis=args.split_vis)
         if args.diff_study > 0:
             r["gradient_difference"] = gradient_difference_study(p, args.diff_study)
         if args.study_iterations > 0:
