@@ -925,7 +925,7 @@ def main() -> int:
                     help="after the run, copy a buffer of this size (for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU split over visibilities is a design note only)", file=sys.stderr)
         return 2
 
