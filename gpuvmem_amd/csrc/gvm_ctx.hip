@@ -330,7 +330,7 @@ static gvm_status create_impl(const gvm_geometry* g, const float* noise_host, gv
   hipDeviceProp_t prop;
   GVM_HIP_TRY(hipGetDeviceProperties(&prop, ctx->device));
   if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0) {
-    set_error(std::string("gvm_create: device is ") + prop.gcnArchName + ", this build targets gfx950");
+    set_error(std::string("gvm_create: device is ") + prop.gcnArchName + ", the library is built for gfx950");
     return GVM_ERR_NODEV;
   }
   ctx->cu_count = prop.multiProcessorCount;
